@@ -1,0 +1,133 @@
+"""cudec_amd -- MI355X (gfx950) batch LZ4 / Snappy block decoding behind the
+cudec C ABI.
+
+The product is the native library cudec_amd/libcudec.so (HIP kernels and
+the C entry points declared in include/cudec.h, include/cudec_snappy.h and
+include/cudec_mi355x.h). This module binds those entry points with ctypes
+under their C names, so Python callers drive exactly the ABI a C caller
+does. Importing it loads the library and fails loudly if it is absent:
+there is no fallback decode path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+# torch is imported first on purpose: its bundled HIP runtime carries the
+# soname libamdhip64.so.7, so loading it before libcudec.so makes the
+# library bind to that same runtime instead of a second copy from
+# /opt/rocm (one process must not hold two HIP runtimes).
+import torch  # noqa: F401
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libcudec.so")
+
+# cudec_status (include/cudec.h)
+CUDEC_OK = 0
+CUDEC_ERR_INVALID_ARGUMENT = 1
+CUDEC_ERR_CORRUPT_INPUT = 2
+CUDEC_ERR_OUTPUT_TOO_SMALL = 3
+CUDEC_ERR_CUDA = 4
+CUDEC_ERR_NOT_IMPLEMENTED = 5
+CUDEC_ERR_UNSUPPORTED = 6
+
+CUDEC_MEM_HOST = 0
+CUDEC_MEM_DEVICE = 1
+
+# The symbols every header in include/ declares, in declaration order.
+ABI_SYMBOLS = {
+    "cudec.h": [
+        "cudec_version", "cudec_lz4_decompress_batch", "cudec_lz4f_decompress",
+        "cudec_stream_ctx_create", "cudec_lz4_decompress_stream_ctx",
+        "cudec_stream_ctx_destroy", "cudec_lz4_decompress_stream",
+    ],
+    "cudec_snappy.h": ["cudec_snappy_decompress_batch"],
+    "cudec_mi355x.h": [
+        "cudec_lz4_decompress_batch_plain", "cudec_snappy_decompress_batch_plain",
+        "cudec_mi355x_max_batch_chunks",
+    ],
+}
+
+
+class ChunkResult(ctypes.Structure):
+    """cudec_chunk_result: {int32 status, uint32 reserved, uint64 bytes_written}."""
+    _fields_ = [("status", ctypes.c_int32), ("reserved", ctypes.c_uint32),
+                ("bytes_written", ctypes.c_uint64)]
+
+
+assert ctypes.sizeof(ChunkResult) == 16 and ChunkResult.bytes_written.offset == 8
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_BATCH_ARGS = [_vp, _vp, _vp, _vp, _sz, _vp, _vp]
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded product library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(the decoders have no non-native fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.cudec_version.restype = ctypes.c_int
+        for name in ("cudec_lz4_decompress_batch", "cudec_snappy_decompress_batch",
+                     "cudec_lz4_decompress_batch_plain",
+                     "cudec_snappy_decompress_batch_plain"):
+            fn = getattr(L, name)
+            fn.argtypes = _BATCH_ARGS
+            fn.restype = ctypes.c_int
+        L.cudec_mi355x_max_batch_chunks.restype = _sz
+        _lib = L
+    return _lib
+
+
+def _batch(name, src_ptrs, src_sizes, dst_ptrs, dst_caps, chunk_count, results, stream):
+    return getattr(lib(), name)(src_ptrs, src_sizes, dst_ptrs, dst_caps,
+                                chunk_count, results, stream)
+
+
+def cudec_version() -> int:
+    return lib().cudec_version()
+
+
+def cudec_lz4_decompress_batch(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
+                               chunk_count, d_results, stream=None) -> int:
+    """include/cudec.h: all arguments are device addresses (ints) except
+    chunk_count; stream is a hipStream_t handle (int) or None."""
+    return _batch("cudec_lz4_decompress_batch", d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                  d_dst_capacities, chunk_count, d_results, stream)
+
+
+def cudec_snappy_decompress_batch(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
+                                  chunk_count, d_results, stream=None) -> int:
+    return _batch("cudec_snappy_decompress_batch", d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                  d_dst_capacities, chunk_count, d_results, stream)
+
+
+def cudec_lz4_decompress_batch_plain(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
+                                     chunk_count, d_results, stream=None) -> int:
+    return _batch("cudec_lz4_decompress_batch_plain", d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                  d_dst_capacities, chunk_count, d_results, stream)
+
+
+def cudec_snappy_decompress_batch_plain(d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                                        d_dst_capacities, chunk_count, d_results,
+                                        stream=None) -> int:
+    return _batch("cudec_snappy_decompress_batch_plain", d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                  d_dst_capacities, chunk_count, d_results, stream)
+
+
+def max_batch_chunks() -> int:
+    return lib().cudec_mi355x_max_batch_chunks()
+
+
+ENTRY = {
+    ("lz4", "tuned"): cudec_lz4_decompress_batch,
+    ("lz4", "plain"): cudec_lz4_decompress_batch_plain,
+    ("snappy", "tuned"): cudec_snappy_decompress_batch,
+    ("snappy", "plain"): cudec_snappy_decompress_batch_plain,
+}

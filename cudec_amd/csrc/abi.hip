@@ -1,0 +1,110 @@
+/* The C ABI entry points of the batch decoders.
+ *
+ * Each entry validates synchronously with no device-runtime call, consumes
+ * the thread's pending runtime error, submits one launch on the caller's
+ * stream, and reports only submission (src/batch.cu:6-31,
+ * src/lz4_decode.cuh:187-201). No exception crosses this boundary. */
+#include "common.h"
+
+#include "cudec_mi355x.h"
+#include "cudec_snappy.h"
+
+namespace cudec_amd {
+
+/* Launch-form selector. */
+enum Form : int { kFormPlain = 0, kFormTuned = 1 };
+
+hipError_t launch_tuned(int format, const void* const* srcs,
+                        const size_t* src_sizes, void* const* dsts,
+                        const size_t* caps, size_t n,
+                        cudec_chunk_result* results, hipStream_t stream);
+
+/* src/lz4_decode.cuh:187-201 with the wave64 bound. */
+static cudec_status validate(const void* const* srcs, const size_t* sizes,
+                             void* const* dsts, const size_t* caps, size_t n,
+                             cudec_chunk_result* results) {
+    if (srcs == nullptr || sizes == nullptr || dsts == nullptr ||
+        caps == nullptr || results == nullptr ||
+        reinterpret_cast<uintptr_t>(results) % 16 != 0 || n == 0 ||
+        n > kMaxBatchChunks) {
+        return CUDEC_ERR_INVALID_ARGUMENT;
+    }
+    return CUDEC_OK;
+}
+
+static cudec_status submit(int format, int form, const void* const* srcs,
+                           const size_t* sizes, void* const* dsts,
+                           const size_t* caps, size_t n,
+                           cudec_chunk_result* results,
+                           cudec_stream_t stream) noexcept {
+    const cudec_status valid = validate(srcs, sizes, dsts, caps, n, results);
+    if (valid != CUDEC_OK) {
+        return valid;
+    }
+    (void)hipGetLastError();
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const hipError_t err =
+        form == kFormPlain
+            ? launch_plain(format, srcs, sizes, dsts, caps, n, results, s)
+            : launch_tuned(format, srcs, sizes, dsts, caps, n, results, s);
+    return err == hipSuccess ? CUDEC_OK : CUDEC_ERR_CUDA;
+}
+
+}  // namespace cudec_amd
+
+using namespace cudec_amd;
+
+extern "C" {
+
+int cudec_version(void) {
+    return CUDEC_VERSION_MAJOR * 10000 + CUDEC_VERSION_MINOR * 100 +
+           CUDEC_VERSION_PATCH;
+}
+
+cudec_status cudec_lz4_decompress_batch(const void* const* d_src_ptrs,
+                                        const size_t* d_src_sizes,
+                                        void* const* d_dst_ptrs,
+                                        const size_t* d_dst_capacities,
+                                        size_t chunk_count,
+                                        cudec_chunk_result* d_results,
+                                        cudec_stream_t stream) {
+    return submit(kFormatLz4, kFormTuned, d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                  d_dst_capacities, chunk_count, d_results, stream);
+}
+
+cudec_status cudec_snappy_decompress_batch(const void* const* d_src_ptrs,
+                                           const size_t* d_src_sizes,
+                                           void* const* d_dst_ptrs,
+                                           const size_t* d_dst_capacities,
+                                           size_t chunk_count,
+                                           cudec_chunk_result* d_results,
+                                           cudec_stream_t stream) {
+    return submit(kFormatSnappy, kFormTuned, d_src_ptrs, d_src_sizes,
+                  d_dst_ptrs, d_dst_capacities, chunk_count, d_results,
+                  stream);
+}
+
+cudec_status cudec_lz4_decompress_batch_plain(const void* const* d_src_ptrs,
+                                              const size_t* d_src_sizes,
+                                              void* const* d_dst_ptrs,
+                                              const size_t* d_dst_capacities,
+                                              size_t chunk_count,
+                                              cudec_chunk_result* d_results,
+                                              cudec_stream_t stream) {
+    return submit(kFormatLz4, kFormPlain, d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                  d_dst_capacities, chunk_count, d_results, stream);
+}
+
+cudec_status cudec_snappy_decompress_batch_plain(
+    const void* const* d_src_ptrs, const size_t* d_src_sizes,
+    void* const* d_dst_ptrs, const size_t* d_dst_capacities,
+    size_t chunk_count, cudec_chunk_result* d_results,
+    cudec_stream_t stream) {
+    return submit(kFormatSnappy, kFormPlain, d_src_ptrs, d_src_sizes,
+                  d_dst_ptrs, d_dst_capacities, chunk_count, d_results,
+                  stream);
+}
+
+size_t cudec_mi355x_max_batch_chunks(void) { return kMaxBatchChunks; }
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+"""ctypes access to the test-side libraries: the CPU oracle (oracle/), the
+optional reference-ladder checker (oracle/_ref/, present only where the
+reference tree was available at build time), and the codec libraries
+(liblz4, libsnappy) used to compress corpora and as parity oracles.
+
+Test infrastructure only; the product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from functools import lru_cache
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+u8p = ctypes.POINTER(ctypes.c_uint8)
+u64 = ctypes.c_uint64
+u64p = ctypes.POINTER(ctypes.c_uint64)
+i32 = ctypes.c_int32
+i32p = ctypes.POINTER(ctypes.c_int32)
+
+OK, INVALID, CORRUPT, TOO_SMALL, DEVICE = 0, 1, 2, 3, 4
+
+
+def _first(names):
+    for n in names:
+        try:
+            return ctypes.CDLL(n)
+        except OSError:
+            continue
+    return None
+
+
+@lru_cache(None)
+def lz4lib():
+    lib = _first(["liblz4.so.1", "/opt/conda/lib/liblz4.so.1",
+                  "/usr/lib/x86_64-linux-gnu/liblz4.so.1"])
+    if lib is None:
+        return None
+    lib.LZ4_compress_default.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.LZ4_compress_default.restype = ctypes.c_int
+    lib.LZ4_decompress_safe.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.LZ4_decompress_safe.restype = ctypes.c_int
+    lib.LZ4_compressBound.argtypes = [ctypes.c_int]
+    lib.LZ4_compressBound.restype = ctypes.c_int
+    lib.LZ4_versionString.restype = ctypes.c_char_p
+    return lib
+
+
+@lru_cache(None)
+def snappylib():
+    lib = _first(["libsnappy.so.1", "/opt/conda/lib/libsnappy.so.1",
+                  "/usr/lib/x86_64-linux-gnu/libsnappy.so.1"])
+    if lib is None:
+        return None
+    lib.snappy_compress.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
+    lib.snappy_compress.restype = ctypes.c_int
+    lib.snappy_uncompress.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
+    lib.snappy_uncompress.restype = ctypes.c_int
+    lib.snappy_max_compressed_length.argtypes = [ctypes.c_size_t]
+    lib.snappy_max_compressed_length.restype = ctypes.c_size_t
+    lib.snappy_uncompressed_length.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    lib.snappy_uncompressed_length.restype = ctypes.c_int
+    lib.snappy_validate_compressed_buffer.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.snappy_validate_compressed_buffer.restype = ctypes.c_int
+    return lib
+
+
+def lz4_compress(data: bytes) -> bytes:
+    """tests/fixtures.cpp:66-82 (LZ4_compress_default, acceleration 1)."""
+    lib = lz4lib()
+    bound = lib.LZ4_compressBound(len(data))
+    out = ctypes.create_string_buffer(bound)
+    w = lib.LZ4_compress_default(data, out, len(data), bound)
+    assert w > 0, "liblz4 compression failed"
+    return out.raw[:w]
+
+
+def lz4_decompress(stream: bytes, capacity: int):
+    """tests/fixtures.cpp:147-162 (OracleDecodes): (accepted, bytes). An
+    empty stream is never handed to liblz4."""
+    lib = lz4lib()
+    if len(stream) == 0:
+        return False, b""
+    buf = ctypes.create_string_buffer(max(capacity, 1))
+    w = lib.LZ4_decompress_safe(stream, buf, len(stream), capacity)
+    if w < 0:
+        return False, b""
+    return True, buf.raw[:w]
+
+
+def snappy_compress(data: bytes) -> bytes:
+    """tests/fixtures.cpp:443-461 (RawCompress via the C API)."""
+    lib = snappylib()
+    cap = lib.snappy_max_compressed_length(len(data))
+    out = ctypes.create_string_buffer(cap)
+    n = ctypes.c_size_t(cap)
+    assert lib.snappy_compress(data, len(data), out, ctypes.byref(n)) == 0
+    return out.raw[:n.value]
+
+
+SNAPPY_ORACLE_MAX_OUTPUT = 256 * 1024 * 1024
+
+
+def snappy_decompress(stream: bytes):
+    """tests/fixtures.cpp:173-198 (SnappyOracleDecodes): exact-consume,
+    exact-produce decode sized from the stream's own declaration, refused
+    above the 256 MiB wrapper bound."""
+    lib = snappylib()
+    if len(stream) == 0:
+        return False, b""
+    declared = ctypes.c_size_t(0)
+    if lib.snappy_uncompressed_length(stream, len(stream), ctypes.byref(declared)) != 0:
+        return False, b""
+    if declared.value > SNAPPY_ORACLE_MAX_OUTPUT:
+        return False, b""
+    buf = ctypes.create_string_buffer(max(declared.value, 1))
+    n = ctypes.c_size_t(declared.value)
+    if lib.snappy_uncompress(stream, len(stream), buf, ctypes.byref(n)) != 0:
+        return False, b""
+    return True, buf.raw[:n.value]
+
+
+# ------------------------------------------------------------------ oracle
+
+@lru_cache(None)
+def oracle():
+    path = os.path.join(ROOT, "oracle", "liboracle.so")
+    lib = ctypes.CDLL(path)
+    lib.oracle_lz4_decode.argtypes = [ctypes.c_char_p, u64, ctypes.c_void_p, u64, u64p]
+    lib.oracle_lz4_decode.restype = i32
+    lib.oracle_lz4_parse.argtypes = [ctypes.c_char_p, u64, u64, u64p, u64p, u64p]
+    lib.oracle_lz4_parse.restype = i32
+    lib.oracle_snappy_declared.argtypes = [ctypes.c_char_p, u64, u64p, u64p, i32p]
+    lib.oracle_snappy_declared.restype = i32
+    lib.oracle_snappy_decode.argtypes = [ctypes.c_char_p, u64, ctypes.c_void_p, u64, u64p, i32p]
+    lib.oracle_snappy_decode.restype = i32
+    lib.oracle_snappy_parse.argtypes = [ctypes.c_char_p, u64, u64, u64p, u64p, u64p, i32p]
+    lib.oracle_snappy_parse.restype = i32
+    lib.oracle_decode_batch_timed.argtypes = [i32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, u64, i32, ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_decode_batch_timed.restype = ctypes.c_double
+    return lib
+
+
+@lru_cache(None)
+def ref():
+    """The reference ladders compiled from the reference tree, or None where
+    that build is absent (e.g. on a host without the reference)."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libcudec_ref.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.ref_lz4_decode.argtypes = [ctypes.c_char_p, u64, ctypes.c_void_p, u64, u64p]
+    lib.ref_lz4_decode.restype = i32
+    lib.ref_lz4_parse.argtypes = [ctypes.c_char_p, u64, u64, u64p]
+    lib.ref_lz4_parse.restype = i32
+    lib.ref_snappy_decode.argtypes = [ctypes.c_char_p, u64, ctypes.c_void_p, u64, u64p, i32p]
+    lib.ref_snappy_decode.restype = i32
+    lib.ref_snappy_declared.argtypes = [ctypes.c_char_p, u64, u64p, u64p, i32p]
+    lib.ref_snappy_declared.restype = i32
+    return lib
+
+
+def _decode(fn, stream: bytes, capacity: int, *extra):
+    buf = ctypes.create_string_buffer(max(capacity, 1))
+    w = u64(0)
+    st = fn(stream, len(stream), buf, capacity, ctypes.byref(w), *extra)
+    return st, (buf.raw[:w.value] if st == OK else b"")
+
+
+def oracle_lz4(stream: bytes, capacity: int):
+    """(status, output) from the C oracle."""
+    return _decode(oracle().oracle_lz4_decode, stream, capacity)
+
+
+def oracle_snappy(stream: bytes, capacity: int):
+    """(status, output, reject rung) from the C oracle."""
+    r = i32(0)
+    st, out = _decode(oracle().oracle_snappy_decode, stream, capacity, ctypes.byref(r))
+    return st, out, r.value
+
+
+def ref_lz4(stream: bytes, capacity: int):
+    return _decode(ref().ref_lz4_decode, stream, capacity)
+
+
+def ref_snappy(stream: bytes, capacity: int):
+    r = i32(0)
+    st, out = _decode(ref().ref_snappy_decode, stream, capacity, ctypes.byref(r))
+    return st, out, r.value
