@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Batch decode benchmark on one MI355X.
+
+One step = one pass of the batch decoder over the whole synthetic batch,
+which is device-resident (compressed arena, pointer / size tables, output
+arena) before the timed region starts.
+
+Default workload: the uneven LZ4 batch -- 20000 chunks, sizes log-uniform
+over [1 KiB, 4 MiB), 80% token-process / 10% uniform-random / 10%
+repetitive chunks, shuffled, compressed with LZ4_compress_default. The
+tuned form (cudec_lz4_decompress_batch) is the measured value; the plain
+one-wave-per-chunk form (cudec_lz4_decompress_batch_plain) is timed on the
+same batch for the "vs plain" ratio. Every chunk of every form is verified
+byte-exact against its original (and status OK, bytes_written == size)
+before timing.
+
+Output: one JSON line (see the keys at the bottom of main()).
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X published HBM3E bandwidth, GB/s
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse_args():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--workload", choices=["uneven", "token64k"], default="uneven")
+    p.add_argument("--format", choices=["lz4", "snappy"], default="lz4")
+    p.add_argument("--chunks", type=int, default=None)
+    p.add_argument("--seed", type=int, default=0x5EED)
+    p.add_argument("--forms", default="tuned,plain")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-threads", type=int, default=None)
+    p.add_argument("--traffic-file", default=None,
+                   help="JSON with measured HBM bytes per launch (rocprofv3 PMC)")
+    return p.parse_args()
+
+
+def build_batch(args):
+    from cudec_amd import workload
+    codec = workload.CODEC_LZ4 if args.format == "lz4" else workload.CODEC_SNAPPY
+    t0 = time.perf_counter()
+    if args.workload == "uneven":
+        n = args.chunks or 20000
+        b = workload.uneven(n, seed=args.seed, codec=codec)
+        name = f"uneven-{args.format}: {n} chunks log-uniform [1 KiB, 4 MiB), 80/10/10 token/random/repetitive"
+    else:
+        n = args.chunks or 65536
+        b = workload.token64k(n, seed=args.seed, codec=codec)
+        name = f"token64k-{args.format}: {n} x 64 KiB token-process chunks"
+    log(f"generated + compressed {n} chunks in {time.perf_counter() - t0:.1f}s: "
+        f"{b.total_uncompressed / 2**30:.2f} GiB -> {b.total_compressed / 2**30:.2f} GiB "
+        f"({workload.codec_version(codec)})")
+    return b, name
+
+
+def time_form(db, fmt, form, steps, warmup, stream, orig_dev, sizes_np):
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        st = db.run(fmt, form, stream=stream)
+        assert st == 0, f"{form} submission failed: {st}"
+    torch.cuda.synchronize()
+    # verify every chunk of this form, byte for byte
+    res = db.results()
+    ok = bool(np.all(res["status"] == 0)) and bool(np.array_equal(res["bytes_written"], sizes_np)) \
+        and bool(np.all(res["reserved"] == 0))
+    same = bool(torch.equal(db.d_out[: orig_dev.numel()], orig_dev))
+    if not (ok and same):
+        bad = int(np.sum(res["status"] != 0))
+        raise SystemExit(f"[bench] {form}: verification FAILED (non-OK chunks: {bad}, "
+                         f"arena equal: {same})")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        st = db.run(fmt, form, stream=stream)
+        ends[k].record(stream)
+        assert st == 0
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return wall, kernel_ms
+
+
+def cpu_baseline(batch, fmt, threads):
+    """The reference ladder compiled from its own source (oracle/_ref) when
+    that build is present, else the C oracle: `threads` host threads, one
+    pass over the whole batch into pre-sized buffers, decode calls only."""
+    import numpy as np
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libcudec_ref.so")
+    port_path = os.path.join(ROOT, "oracle", "liboracle.so")
+    u64 = ctypes.c_uint64
+    if os.path.exists(ref_path):
+        lib = ctypes.CDLL(ref_path)
+        fn = lib.ref_lz4_decode if fmt == "lz4" else lib.ref_snappy_decode
+        kind = "reference"
+    else:
+        lib = ctypes.CDLL(port_path)
+        fn = lib.oracle_lz4_decode if fmt == "lz4" else lib.oracle_snappy_decode
+        kind = "port"
+    extra = [] if fmt == "lz4" else [ctypes.c_void_p]
+    fn.argtypes = [ctypes.c_void_p, u64, ctypes.c_void_p, u64, ctypes.POINTER(u64)] + extra
+    fn.restype = ctypes.c_int32
+    out = np.empty(batch.total_uncompressed, dtype=np.uint8)
+    cbase = batch.comp.ctypes.data
+    obase = out.ctypes.data
+    n = batch.n
+    bounds = [n * t // threads for t in range(threads + 1)]
+
+    def work(t):
+        w = u64(0)
+        bad = 0
+        for i in range(bounds[t], bounds[t + 1]):
+            args = [cbase + int(batch.comp_off[i]), int(batch.comp_size[i]),
+                    obase + int(batch.orig_off[i]), int(batch.sizes[i]), ctypes.byref(w)]
+            if extra:
+                args.append(None)
+            if fn(*args) != 0 or w.value != int(batch.sizes[i]):
+                bad += 1
+        return bad
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        t0 = time.perf_counter()
+        bad = sum(ex.map(work, range(threads)))
+        secs = time.perf_counter() - t0
+    if bad or not np.array_equal(out, batch.orig):
+        raise SystemExit("[bench] CPU baseline produced wrong output")
+    return {"value": round(batch.total_uncompressed / secs / 1e9, 3), "unit": "GB/s (uncompressed)",
+            "cores": threads, "kind": kind,
+            "sample": f"one pass over the full batch ({n} chunks, "
+                      f"{batch.total_uncompressed / 2**30:.2f} GiB out), {threads} threads, "
+                      f"{secs:.2f} s wall"}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(chunks shard across GPUs with no exchange; see DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from cudec_amd.device import DeviceBatch
+
+    torch.cuda.init()
+    batch, workload_name = build_batch(args)
+    t0 = time.perf_counter()
+    db = DeviceBatch(batch.comp, batch.comp_off, batch.comp_size, batch.sizes,
+                     out_off=batch.orig_off)
+    orig_dev = torch.from_numpy(batch.orig).to("cuda")
+    torch.cuda.synchronize()
+    log(f"uploaded in {time.perf_counter() - t0:.1f}s")
+    stream = torch.cuda.Stream()  # default priority
+    uncomp = batch.total_uncompressed
+    algo = uncomp + batch.total_compressed
+    forms = [f for f in args.forms.split(",") if f]
+    measured = {}
+    for form in forms:
+        wall, kms = time_form(db, args.format, form, args.steps, args.warmup, stream, orig_dev,
+                              batch.sizes)
+        avg_ms = sum(kms) / len(kms)
+        measured[form] = {"wall_s": wall, "kernel_ms": kms, "avg_kernel_ms": avg_ms,
+                          "gbps": uncomp * args.steps / wall / 1e9,
+                          "kernel_gbps": uncomp / (avg_ms / 1e3) / 1e9,
+                          "algo_gbps": algo / (avg_ms / 1e3) / 1e9}
+        log(f"{form}: {avg_ms:.3f} ms/launch (event), {measured[form]['gbps']:.1f} GB/s "
+            f"uncompressed (wall), {measured[form]['algo_gbps']:.1f} GB/s algorithmic "
+            f"= {100 * measured[form]['algo_gbps'] / HBM_PEAK_GBPS:.1f}% of HBM peak")
+    head = measured[forms[0]]
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(batch, args.format, threads)
+        log(f"cpu baseline: {cpu}")
+    traffic = None
+    if args.traffic_file and os.path.exists(args.traffic_file):
+        with open(args.traffic_file) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    line = {
+        "metric": "LZ4 batch decode GB/s (uncompressed) + % HBM peak, vs plain; p99 request latency"
+        if args.format == "lz4" else "Snappy batch decode GB/s (uncompressed) + % HBM peak, vs plain",
+        "value": round(head["gbps"], 3),
+        "unit": "GB/s (uncompressed)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * head["wall_s"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded generator, liblz4/snappy-compressed on the host)",
+        "config": {"workload": workload_name, "chunks": batch.n, "form": forms[0],
+                   "uncompressed_bytes": uncomp, "compressed_bytes": batch.total_compressed,
+                   "seed": args.seed},
+        "roofline": {"bound": "hbm", "achieved": round(head["algo_gbps"], 2),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(head["algo_gbps"] / HBM_PEAK_GBPS, 5),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    if "plain" in measured and forms[0] != "plain":
+        line["plain"] = {"value": round(measured["plain"]["gbps"], 3),
+                         "avg_kernel_ms": round(measured["plain"]["avg_kernel_ms"], 4),
+                         "algo_gbps": round(measured["plain"]["algo_gbps"], 2)}
+        line["speedup_vs_plain"] = round(measured["plain"]["avg_kernel_ms"] / head["avg_kernel_ms"], 3)
+    line["avg_kernel_ms"] = round(head["avg_kernel_ms"], 4)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -212,33 +212,43 @@ def test_generated_workloads_exact(form):
 
 
 def test_forms_and_streams_deterministic(lz4_fixtures):
-    """tests/determinism_gpu.cu:163-265: the whole output arena and every
-    result record are identical across both launch forms, repeated runs,
-    and a three-stream split of the batch."""
+    """tests/determinism_gpu.cu:163-265 under docs/DETERMINISM.md:20-32,63-68:
+    every result record, and the whole destination slot of every chunk that
+    decoded OK (its bytes and the untouched poison beyond them), are
+    identical across both launch forms, repeated runs, and a three-stream
+    split of the batch. Rejected chunks' leftover bytes are outside the
+    promise and are not compared."""
     import torch
 
+    import cudec_amd
     from cudec_amd import workload
     from cudec_amd.device import DeviceBatch
     b = workload.uneven(90, seed=0x79, min_size=1024, octaves=7)
     streams = [b.compressed(i) for i in range(b.n)]
     streams += [m.stream for f in lz4_fixtures[:6] for m in corpus.mutate_stream(f.compressed, f.seed)]
     caps = [int(s) + 3 for s in b.sizes] + [70000] * (len(streams) - b.n)
-    ref_arena = ref_res = None
+
+    def ok_slots(db, res, arena):
+        return [db.slot(arena, i).tobytes() if res["status"][i] == 0 else None
+                for i in range(db.n)]
+
+    ref_res = ref_slots = None
     for form in FORMS:
         for _ in range(3):
             db, res, arena = run_device("lz4", form, streams, caps, poison=0x3C)
-            if ref_arena is None:
-                ref_arena, ref_res = arena, res
-            assert np.array_equal(arena, ref_arena)
-            assert np.array_equal(res, ref_res)
-    # three streams, each decoding a third of the chunks of one batch
+            slots = ok_slots(db, res, arena)
+            if ref_res is None:
+                ref_res, ref_slots = res, slots
+                assert np.sum(res["status"] == 0) > b.n // 2
+                assert np.sum(res["status"] != 0) > 0
+            assert np.array_equal(res, ref_res), form
+            assert slots == ref_slots, form
     db = DeviceBatch.from_streams(streams, caps, poison=0x3C)
     db.reset_results()
     torch.cuda.synchronize()
     n = db.n
     cuts = [0, n // 3, 2 * n // 3, n]
     ss = [torch.cuda.Stream() for _ in range(3)]
-    import cudec_amd
     for k in range(3):
         lo, hi = cuts[k], cuts[k + 1]
         st = cudec_amd.cudec_lz4_decompress_batch(
@@ -247,5 +257,6 @@ def test_forms_and_streams_deterministic(lz4_fixtures):
             db.d_results.data_ptr() + 16 * lo, ss[k].cuda_stream)
         assert st == 0
     torch.cuda.synchronize()
-    assert np.array_equal(db.output_arena(), ref_arena)
-    assert np.array_equal(db.results(), ref_res)
+    res = db.results()
+    assert np.array_equal(res, ref_res)
+    assert ok_slots(db, res, db.output_arena()) == ref_slots
