@@ -81,6 +81,16 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = _BATCH_ARGS
             fn.restype = ctypes.c_int
         L.cudec_mi355x_max_batch_chunks.restype = _sz
+        L.cudec_stream_ctx_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        L.cudec_stream_ctx_create.restype = ctypes.c_int
+        L.cudec_stream_ctx_destroy.argtypes = [_vp]
+        L.cudec_stream_ctx_destroy.restype = None
+        L.cudec_lz4_decompress_stream_ctx.argtypes = [_vp, _vp, _vp, _vp, _vp, _sz, ctypes.c_int, _vp]
+        L.cudec_lz4_decompress_stream_ctx.restype = ctypes.c_int
+        L.cudec_lz4_decompress_stream.argtypes = [_vp, _vp, _vp, _vp, _sz, ctypes.c_int, _vp]
+        L.cudec_lz4_decompress_stream.restype = ctypes.c_int
+        L.cudec_lz4f_decompress.argtypes = [_vp, _sz, _vp, _sz, ctypes.POINTER(_sz)]
+        L.cudec_lz4f_decompress.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -131,3 +141,43 @@ ENTRY = {
     ("snappy", "tuned"): cudec_snappy_decompress_batch,
     ("snappy", "plain"): cudec_snappy_decompress_batch_plain,
 }
+
+
+# ----------------------------------------------------------- streaming context
+
+def cudec_stream_ctx_create():
+    """Returns (status, handle)."""
+    h = ctypes.c_void_p(0)
+    st = lib().cudec_stream_ctx_create(ctypes.byref(h))
+    return st, h.value
+
+
+def cudec_stream_ctx_destroy(ctx) -> None:
+    lib().cudec_stream_ctx_destroy(ctx)
+
+
+def cudec_lz4_decompress_stream_ctx(ctx, h_src_ptrs, h_src_sizes, dst_ptrs, dst_caps,
+                                    chunk_count, dst_space, h_results) -> int:
+    """include/cudec.h:193-197; array arguments are host addresses (ints)."""
+    return lib().cudec_lz4_decompress_stream_ctx(ctx, h_src_ptrs, h_src_sizes, dst_ptrs,
+                                                 dst_caps, chunk_count, dst_space, h_results)
+
+
+def cudec_lz4_decompress_stream(h_src_ptrs, h_src_sizes, dst_ptrs, dst_caps, chunk_count,
+                                dst_space, h_results) -> int:
+    return lib().cudec_lz4_decompress_stream(h_src_ptrs, h_src_sizes, dst_ptrs, dst_caps,
+                                             chunk_count, dst_space, h_results)
+
+
+# ------------------------------------------------------------------ LZ4 frames
+
+def cudec_lz4f_decompress(frame: bytes, capacity: int):
+    """include/cudec.h (cudec_lz4f_decompress): one LZ4 frame in host memory
+    decoded into a host buffer of `capacity` bytes. Returns (status, bytes):
+    the decoded content on CUDEC_OK, b"" otherwise."""
+    out = ctypes.create_string_buffer(max(capacity, 1))
+    written = _sz(0)
+    src = ctypes.c_char_p(frame)
+    st = lib().cudec_lz4f_decompress(ctypes.cast(src, _vp), len(frame), ctypes.cast(out, _vp),
+                                     capacity, ctypes.byref(written))
+    return st, (out.raw[:written.value] if st == CUDEC_OK else b"")

@@ -1,0 +1,323 @@
+/* cudec_lz4f_decompress: one LZ4 frame (the .lz4 container), host memory
+ * to host memory, through the batch block decoder.
+ *
+ * Contract: include/cudec.h:110-132; behaviour follows iderex/cudec's
+ * src/frame.cpp:282-422 (parse, status mapping) and 53-278 (decode and
+ * assembly). Format: the public LZ4 frame format (magic 0x184D2204, FLG,
+ * BD, optional content size, header checksum byte, data blocks with the
+ * high bit marking an uncompressed block, optional block checksums, end
+ * mark, optional content checksum; checksums are XXH32 with seed 0).
+ *
+ * Decoding here: every compressed block of the frame goes to the device in
+ * one upload and one batch launch (one destination slot of the frame's
+ * block maximum per block), and the decoded blocks come back with one copy
+ * per run of blocks that are contiguous both on the device and in the
+ * output -- every block but the last decodes to exactly the block maximum,
+ * so a frame of compressed blocks comes back in one copy. The caller's
+ * capacity is checked before any output byte is written. */
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "cudec.h"
+#include "hip_owner.h"
+
+namespace cudec_amd {
+namespace {
+
+constexpr uint32_t kFrameMagic = 0x184D2204u;
+
+uint32_t le32(const unsigned char* p) {
+    uint32_t v;
+    std::memcpy(&v, p, 4);
+    return v;
+}
+
+uint64_t le64(const unsigned char* p) {
+    uint64_t v;
+    std::memcpy(&v, p, 8);
+    return v;
+}
+
+/* XXH32, seed 0, from the published xxHash specification: four 32-bit
+ * accumulators over 16-byte stripes, then 4-byte and 1-byte tail rounds
+ * and the final avalanche. */
+constexpr uint32_t kP1 = 0x9E3779B1u, kP2 = 0x85EBCA77u, kP3 = 0xC2B2AE3Du,
+                   kP4 = 0x27D4EB2Fu, kP5 = 0x165667B1u;
+
+inline uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+uint32_t xxh32(const unsigned char* p, size_t len) {
+    const unsigned char* const end = p + len;
+    uint32_t h;
+    if (len >= 16) {
+        uint32_t a = kP1 + kP2, b = kP2, c = 0, d = 0u - kP1;
+        const unsigned char* const last = end - 16;
+        do {
+            a = rotl32(a + le32(p) * kP2, 13) * kP1;
+            b = rotl32(b + le32(p + 4) * kP2, 13) * kP1;
+            c = rotl32(c + le32(p + 8) * kP2, 13) * kP1;
+            d = rotl32(d + le32(p + 12) * kP2, 13) * kP1;
+            p += 16;
+        } while (p <= last);
+        h = rotl32(a, 1) + rotl32(b, 7) + rotl32(c, 12) + rotl32(d, 18);
+    } else {
+        h = kP5;
+    }
+    h += static_cast<uint32_t>(len);
+    while (p + 4 <= end) {
+        h = rotl32(h + le32(p) * kP3, 17) * kP4;
+        p += 4;
+    }
+    while (p < end) {
+        h = rotl32(h + static_cast<uint32_t>(*p) * kP5, 11) * kP1;
+        p++;
+    }
+    h ^= h >> 15;
+    h *= kP2;
+    h ^= h >> 13;
+    h *= kP3;
+    h ^= h >> 16;
+    return h;
+}
+
+struct Block {
+    size_t off;
+    size_t len;
+    bool stored;  /* the high bit: the block is stored uncompressed */
+};
+
+/* Decodes the compressed blocks in one batch, then assembles the output. */
+cudec_status decode_blocks(const unsigned char* f, const std::vector<Block>& blocks,
+                           size_t block_max, unsigned char* out, size_t cap,
+                           size_t* total_out) {
+    std::vector<size_t> ci;
+    size_t total_src = 0;
+    for (size_t i = 0; i < blocks.size(); i++) {
+        if (!blocks[i].stored) {
+            ci.push_back(i);
+            total_src += blocks[i].len;
+        }
+    }
+    const size_t n = ci.size();
+    std::vector<size_t> decoded(blocks.size(), 0);
+    StreamOwner stream;
+    DeviceBuffer d_src, d_dst, d_meta, d_res;
+    if (n != 0) {
+        if (n > SIZE_MAX / block_max || n > SIZE_MAX / (4 * sizeof(void*))) {
+            return CUDEC_ERR_CORRUPT_INPUT;
+        }
+        if (stream.create() != hipSuccess || d_src.ensure(total_src) != hipSuccess ||
+            d_dst.ensure(n * block_max) != hipSuccess ||
+            d_meta.ensure(4 * n * sizeof(void*)) != hipSuccess ||
+            d_res.ensure(n * sizeof(cudec_chunk_result)) != hipSuccess) {
+            return CUDEC_ERR_CUDA;
+        }
+        std::vector<unsigned char> stage(total_src);
+        const size_t stride = n * sizeof(void*);
+        std::vector<unsigned char> meta(4 * stride);
+        unsigned char* ds = static_cast<unsigned char*>(d_src.p);
+        unsigned char* dd = static_cast<unsigned char*>(d_dst.p);
+        size_t so = 0;
+        for (size_t k = 0; k < n; k++) {
+            const Block& b = blocks[ci[k]];
+            std::memcpy(stage.data() + so, f + b.off, b.len);
+            const void* sp = ds + so;
+            void* dp = dd + k * block_max;
+            std::memcpy(meta.data() + k * sizeof(void*), &sp, sizeof(void*));
+            std::memcpy(meta.data() + stride + k * sizeof(size_t), &b.len, sizeof(size_t));
+            std::memcpy(meta.data() + 2 * stride + k * sizeof(void*), &dp, sizeof(void*));
+            std::memcpy(meta.data() + 3 * stride + k * sizeof(size_t), &block_max, sizeof(size_t));
+            so += b.len;
+        }
+        hipStream_t st = stream.s;
+        if (hipMemcpyAsync(ds, stage.data(), total_src, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_meta.p, meta.data(), meta.size(), hipMemcpyHostToDevice, st) !=
+                hipSuccess) {
+            (void)hipStreamSynchronize(st);
+            return CUDEC_ERR_CUDA;
+        }
+        unsigned char* dm = static_cast<unsigned char*>(d_meta.p);
+        const cudec_status launched = cudec_lz4_decompress_batch(
+            reinterpret_cast<const void* const*>(dm), reinterpret_cast<const size_t*>(dm + stride),
+            reinterpret_cast<void* const*>(dm + 2 * stride),
+            reinterpret_cast<const size_t*>(dm + 3 * stride), n,
+            static_cast<cudec_chunk_result*>(d_res.p), reinterpret_cast<cudec_stream_t>(st));
+        std::vector<cudec_chunk_result> res(n);
+        const bool copied = launched == CUDEC_OK &&
+                            hipMemcpyAsync(res.data(), d_res.p, n * sizeof(cudec_chunk_result),
+                                           hipMemcpyDeviceToHost, st) == hipSuccess;
+        /* the staging vectors must outlive the asynchronous uploads */
+        if (hipStreamSynchronize(st) != hipSuccess) {
+            return CUDEC_ERR_CUDA;
+        }
+        if (launched != CUDEC_OK) {
+            return launched;
+        }
+        if (!copied) {
+            return CUDEC_ERR_CUDA;
+        }
+        for (size_t k = 0; k < n; k++) {
+            /* any block the decoder refuses (including one longer than the
+             * block maximum) makes the frame corrupt */
+            if (res[k].status != CUDEC_OK || res[k].bytes_written > block_max) {
+                return CUDEC_ERR_CORRUPT_INPUT;
+            }
+            decoded[ci[k]] = static_cast<size_t>(res[k].bytes_written);
+        }
+    }
+    size_t total = 0;
+    for (size_t i = 0; i < blocks.size(); i++) {
+        const size_t len = blocks[i].stored ? blocks[i].len : decoded[i];
+        if (len > cap - total) {
+            return CUDEC_ERR_OUTPUT_TOO_SMALL;
+        }
+        total += len;
+    }
+    size_t run_dev = 0, run_out = 0, run_len = 0, pos = 0, k = 0;
+    hipError_t err = hipSuccess;
+    for (size_t i = 0; i < blocks.size() && err == hipSuccess; i++) {
+        if (blocks[i].stored) {
+            if (blocks[i].len != 0) {
+                std::memcpy(out + pos, f + blocks[i].off, blocks[i].len);
+            }
+            pos += blocks[i].len;
+            continue;
+        }
+        const size_t len = decoded[i];
+        const size_t dev = k * block_max;
+        if (run_len != 0 && run_dev + run_len == dev && run_out + run_len == pos) {
+            run_len += len;
+        } else {
+            if (run_len != 0) {
+                err = hipMemcpyAsync(out + run_out, static_cast<unsigned char*>(d_dst.p) + run_dev,
+                                     run_len, hipMemcpyDeviceToHost, stream.s);
+            }
+            run_dev = dev;
+            run_out = pos;
+            run_len = len;
+        }
+        pos += len;
+        k++;
+    }
+    if (err == hipSuccess && run_len != 0) {
+        err = hipMemcpyAsync(out + run_out, static_cast<unsigned char*>(d_dst.p) + run_dev, run_len,
+                             hipMemcpyDeviceToHost, stream.s);
+    }
+    if (n != 0 && hipStreamSynchronize(stream.s) != hipSuccess) {
+        err = hipErrorUnknown;
+    }
+    if (err != hipSuccess) {
+        return CUDEC_ERR_CUDA;
+    }
+    *total_out = total;
+    return CUDEC_OK;
+}
+
+cudec_status decode_frame(const unsigned char* f, size_t size, unsigned char* out, size_t cap,
+                          size_t* written) {
+    if (size < 7 || le32(f) != kFrameMagic) {
+        return CUDEC_ERR_CORRUPT_INPUT;
+    }
+    const unsigned flg = f[4], bd = f[5];
+    if (((flg >> 6) & 3) != 1 || (flg & 2) != 0 || (bd & 0x8F) != 0) {
+        return CUDEC_ERR_CORRUPT_INPUT; /* version, reserved bits */
+    }
+    const unsigned bmax = (bd >> 4) & 7;
+    if (bmax < 4) {
+        return CUDEC_ERR_CORRUPT_INPUT;
+    }
+    const size_t block_max = size_t{64 << 10} << ((bmax - 4) * 2);
+    const bool independent = (flg >> 5) & 1, block_sum = (flg >> 4) & 1,
+               has_size = (flg >> 3) & 1, content_sum = (flg >> 2) & 1, dict = flg & 1;
+    if (!independent || dict) {
+        return CUDEC_ERR_UNSUPPORTED;
+    }
+    size_t pos = 6;
+    uint64_t declared = 0;
+    if (has_size) {
+        if (size < pos + 8) {
+            return CUDEC_ERR_CORRUPT_INPUT;
+        }
+        declared = le64(f + pos);
+        pos += 8;
+    }
+    if (size < pos + 1) {
+        return CUDEC_ERR_CORRUPT_INPUT;
+    }
+    if (((xxh32(f + 4, pos - 4) >> 8) & 0xFF) != f[pos]) {
+        return CUDEC_ERR_CORRUPT_INPUT;
+    }
+    pos++;
+    std::vector<Block> blocks;
+    uint64_t fuel = size / 4 + 1; /* each block consumes at least 4 bytes */
+    bool ended = false;
+    while (fuel-- != 0) {
+        if (size < pos + 4) {
+            return CUDEC_ERR_CORRUPT_INPUT;
+        }
+        const uint32_t bs = le32(f + pos);
+        pos += 4;
+        if (bs == 0) {
+            ended = true;
+            break;
+        }
+        const size_t len = bs & 0x7FFFFFFFu;
+        if (len == 0 || len > block_max || size - pos < len) {
+            return CUDEC_ERR_CORRUPT_INPUT;
+        }
+        if (block_sum) {
+            if (size - pos - len < 4 || xxh32(f + pos, len) != le32(f + pos + len)) {
+                return CUDEC_ERR_CORRUPT_INPUT;
+            }
+        }
+        blocks.push_back(Block{pos, len, ((bs >> 31) & 1) != 0});
+        pos += len + (block_sum ? 4 : 0);
+    }
+    if (!ended) {
+        return CUDEC_ERR_CORRUPT_INPUT;
+    }
+    size_t total = 0;
+    const cudec_status st = decode_blocks(f, blocks, block_max, out, cap, &total);
+    if (st != CUDEC_OK) {
+        return st;
+    }
+    if (has_size && declared != total) {
+        return CUDEC_ERR_CORRUPT_INPUT;
+    }
+    if (content_sum) {
+        if (size < pos + 4 || xxh32(out, total) != le32(f + pos)) {
+            return CUDEC_ERR_CORRUPT_INPUT;
+        }
+    }
+    *written = total;
+    return CUDEC_OK;
+}
+
+}  // namespace
+}  // namespace cudec_amd
+
+extern "C" cudec_status cudec_lz4f_decompress(const void* frame, size_t frame_size, void* dst,
+                                              size_t dst_capacity, size_t* bytes_written) {
+    if (bytes_written != nullptr) {
+        *bytes_written = 0;
+    }
+    if (frame == nullptr || bytes_written == nullptr || (dst == nullptr && dst_capacity != 0)) {
+        return CUDEC_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        size_t w = 0;
+        const cudec_status st = cudec_amd::decode_frame(static_cast<const unsigned char*>(frame),
+                                                        frame_size,
+                                                        static_cast<unsigned char*>(dst),
+                                                        dst_capacity, &w);
+        if (st == CUDEC_OK) {
+            *bytes_written = w;
+        }
+        return st;
+    } catch (...) {
+        return CUDEC_ERR_CUDA;
+    }
+}

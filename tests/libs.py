@@ -189,3 +189,127 @@ def ref_snappy(stream: bytes, capacity: int):
     r = i32(0)
     st, out = _decode(ref().ref_snappy_decode, stream, capacity, ctypes.byref(r))
     return st, out, r.value
+
+
+# ------------------------------------------------------------- LZ4 frames
+
+class LZ4FFrameInfo(ctypes.Structure):
+    _fields_ = [("blockSizeID", ctypes.c_int), ("blockMode", ctypes.c_int),
+                ("contentChecksumFlag", ctypes.c_int), ("frameType", ctypes.c_int),
+                ("contentSize", ctypes.c_ulonglong), ("dictID", ctypes.c_uint),
+                ("blockChecksumFlag", ctypes.c_int)]
+
+
+class LZ4FPreferences(ctypes.Structure):
+    _fields_ = [("frameInfo", LZ4FFrameInfo), ("compressionLevel", ctypes.c_int),
+                ("autoFlush", ctypes.c_uint), ("favorDecSpeed", ctypes.c_uint),
+                ("reserved", ctypes.c_uint * 3)]
+
+
+def lz4_frame(data: bytes, block_size_id: int = 4, independent: bool = True,
+              content_checksum: bool = False, block_checksum: bool = False,
+              content_size: bool = False) -> bytes:
+    """liblz4's LZ4F_compressFrame (block size id 4..7 = 64 KiB..4 MiB)."""
+    lib = lz4lib()
+    lib.LZ4F_compressFrameBound.argtypes = [ctypes.c_size_t, ctypes.c_void_p]
+    lib.LZ4F_compressFrameBound.restype = ctypes.c_size_t
+    lib.LZ4F_compressFrame.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p,
+                                       ctypes.c_size_t, ctypes.c_void_p]
+    lib.LZ4F_compressFrame.restype = ctypes.c_size_t
+    prefs = LZ4FPreferences()
+    prefs.frameInfo.blockSizeID = block_size_id
+    prefs.frameInfo.blockMode = 1 if independent else 0
+    prefs.frameInfo.contentChecksumFlag = int(content_checksum)
+    prefs.frameInfo.blockChecksumFlag = int(block_checksum)
+    prefs.frameInfo.contentSize = len(data) if content_size else 0
+    bound = lib.LZ4F_compressFrameBound(len(data), ctypes.byref(prefs))
+    out = ctypes.create_string_buffer(bound)
+    n = lib.LZ4F_compressFrame(out, bound, data, len(data), ctypes.byref(prefs))
+    assert n < (1 << 63), "LZ4F_compressFrame failed"
+    return out.raw[:n]
+
+
+def lz4_frame_decode(frame: bytes, capacity: int):
+    """liblz4's streaming frame API (LZ4F_decompress) as the frame oracle:
+    (accepted, bytes). A frame that ends before its end mark, or that needs
+    more than `capacity` bytes of output, is refused."""
+    lib = lz4lib()
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    lib.LZ4F_createDecompressionContext.argtypes = [ctypes.POINTER(vp), ctypes.c_uint]
+    lib.LZ4F_createDecompressionContext.restype = sz
+    lib.LZ4F_freeDecompressionContext.argtypes = [vp]
+    lib.LZ4F_freeDecompressionContext.restype = sz
+    lib.LZ4F_decompress.argtypes = [vp, vp, ctypes.POINTER(sz), vp, ctypes.POINTER(sz), vp]
+    lib.LZ4F_decompress.restype = sz
+    lib.LZ4F_isError.argtypes = [sz]
+    lib.LZ4F_isError.restype = ctypes.c_uint
+    dctx = vp()
+    if lib.LZ4F_isError(lib.LZ4F_createDecompressionContext(ctypes.byref(dctx), 100)):
+        return False, b""
+    src = ctypes.create_string_buffer(frame, max(len(frame), 1))
+    out = ctypes.create_string_buffer(max(capacity, 1))
+    so = do = 0
+    hint = 1
+    ok = True
+    try:
+        while so < len(frame):
+            sl, dl = sz(len(frame) - so), sz(capacity - do)
+            hint = lib.LZ4F_decompress(dctx, ctypes.addressof(out) + do, ctypes.byref(dl),
+                                       ctypes.addressof(src) + so, ctypes.byref(sl), None)
+            if lib.LZ4F_isError(hint):
+                ok = False
+                break
+            so += sl.value
+            do += dl.value
+            if hint == 0:
+                break
+            if sl.value == 0 and dl.value == 0:
+                ok = False
+                break
+    finally:
+        lib.LZ4F_freeDecompressionContext(dctx)
+    if ok and hint != 0:
+        ok = False
+    return ok, (out.raw[:do] if ok else b"")
+
+
+
+_P1, _P2, _P3, _P4, _P5 = 0x9E3779B1, 0x85EBCA77, 0xC2B2AE3D, 0x27D4EB2F, 0x165667B1
+_M32 = 0xFFFFFFFF
+
+
+def _rotl(x: int, r: int) -> int:
+    return ((x << r) | (x >> (32 - r))) & _M32
+
+
+def xxh32(data: bytes, seed: int = 0) -> int:
+    """XXH32 from the published xxHash specification (four 32-bit lanes
+    over 16-byte stripes, then 4-byte and 1-byte tails and the avalanche).
+    Used to repair header checksums of hand-mutated frames; pinned against
+    the header-checksum byte liblz4 writes and the specification's
+    empty-input value."""
+    n = len(data)
+    i = 0
+    if n >= 16:
+        v = [(seed + _P1 + _P2) & _M32, (seed + _P2) & _M32, seed & _M32, (seed - _P1) & _M32]
+        while i + 16 <= n:
+            for k in range(4):
+                lane = int.from_bytes(data[i + 4 * k:i + 4 * k + 4], "little")
+                v[k] = (_rotl((v[k] + lane * _P2) & _M32, 13) * _P1) & _M32
+            i += 16
+        h = (_rotl(v[0], 1) + _rotl(v[1], 7) + _rotl(v[2], 12) + _rotl(v[3], 18)) & _M32
+    else:
+        h = (seed + _P5) & _M32
+    h = (h + n) & _M32
+    while i + 4 <= n:
+        h = (_rotl((h + int.from_bytes(data[i:i + 4], "little") * _P3) & _M32, 17) * _P4) & _M32
+        i += 4
+    while i < n:
+        h = (_rotl((h + data[i] * _P5) & _M32, 11) * _P1) & _M32
+        i += 1
+    h ^= h >> 15
+    h = (h * _P2) & _M32
+    h ^= h >> 13
+    h = (h * _P3) & _M32
+    h ^= h >> 16
+    return h
