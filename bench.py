@@ -50,6 +50,17 @@ def parse_args():
     p.add_argument("--cpu-threads", type=int, default=None)
     p.add_argument("--traffic-file", default=None,
                    help="JSON with measured HBM bytes per launch (rocprofv3 PMC)")
+    p.add_argument("--copy-probe-mib", type=int, default=0,
+                   help="before timing, run a plain elementwise copy (torch.neg over int32) "
+                        "of this many MiB three times: the known-size kernel that HBM "
+                        "counters are scaled by")
+    p.add_argument("--serve-rates", default="2000,8000,32000",
+                   help="request rates (per second) of the serving leg; empty to skip it")
+    p.add_argument("--serve-seconds", type=float, default=0.5,
+                   help="scheduled arrival span of each serving run")
+    p.add_argument("--serve-pool", type=int, default=2048,
+                   help="distinct chunks the serving requests draw from")
+    p.add_argument("--serve-budget-gib", type=float, default=64.0)
     return p.parse_args()
 
 
@@ -152,6 +163,60 @@ def cpu_baseline(batch, fmt, threads):
                       f"{secs:.2f} s wall"}
 
 
+def serving_leg(args):
+    """Poisson request streams through the native serving loop. Per rate:
+    one run with device digests, checked chunk by chunk against the
+    originals, then the timed run (no digests) whose latencies are
+    reported: nearest-rank p50/p99 from scheduled arrival to completion."""
+    import numpy as np
+    import torch
+
+    from cudec_amd import serve
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    t0 = time.perf_counter()
+    pool = serve.chunk_pool(args.serve_pool, seed=args.seed ^ 0x5E5E)
+    want = np.array([serve.host_digest(pool.orig[int(o):int(o) + int(n)])
+                     for o, n in zip(pool.orig_off, pool.sizes)], dtype=np.uint64)
+    d_comp = torch.from_numpy(pool.comp.copy()).to("cuda")
+    torch.cuda.synchronize()
+    log(f"serving pool: {pool.n} chunks, {pool.total_uncompressed / 2**20:.0f} MiB out, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    budget = int(args.serve_budget_gib * 2**30)
+    out = []
+    for k, rate in enumerate(rates):
+        nreq = max(16, int(rate * args.serve_seconds))
+        req, ids = serve.make_trace(rate, nreq, pool.n, seed=args.seed + k)
+        check = serve.serve(d_comp.data_ptr(), pool, req, ids, budget=budget, inflight=4,
+                            digests=True)
+        ok = check.status == 0 and bool(np.all(check.results["status"] == 0)) and \
+            bool(np.array_equal(check.results["bytes_written"], pool.sizes[ids])) and \
+            bool(np.array_equal(check.digests, want[ids]))
+        if not ok:
+            raise SystemExit(f"[bench] serving at {rate:.0f}/s: verification FAILED")
+        run = serve.serve(d_comp.data_ptr(), pool, req, ids, budget=budget, inflight=4,
+                          digests=False)
+        if run.status != 0 or not np.array_equal(run.results, check.results):
+            raise SystemExit(f"[bench] serving at {rate:.0f}/s: timed run differs")
+        st = run.stats
+        algo = st.output_bytes + st.compressed_bytes
+        row = {"rate_per_s": rate, "requests": nreq, "chunks": int(len(ids)),
+               "p50_ms": round(1e3 * serve.percentile(run.latency_s, 50), 3),
+               "p99_ms": round(1e3 * serve.percentile(run.latency_s, 99), 3),
+               "gbps_uncompressed": round(st.output_bytes / st.elapsed_s / 1e9, 2),
+               "gbps_algorithmic": round(algo / st.elapsed_s / 1e9, 2),
+               "offered_gbps_uncompressed": round(st.output_bytes / max(
+                   float(req["arrival_s"][-1]), 1e-9) / 1e9, 2),
+               "launches": int(st.launches), "max_batch_requests": int(st.max_batch_requests),
+               "max_inflight": int(st.max_inflight),
+               "peak_pool_mib": round(st.peak_pool_bytes / 2**20, 1)}
+        log(f"serving {rate:.0f}/s: {row}")
+        out.append(row)
+    return {"budget_bytes": budget, "inflight": 4, "verified": True,
+            "workload": f"{pool.n} distinct token chunks, log-uniform [4 KiB, 1 MiB); "
+                        "requests of 1-256 chunks (log-uniform), Poisson arrivals",
+            "rates": out}
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -172,6 +237,16 @@ def main():
     torch.cuda.synchronize()
     log(f"uploaded in {time.perf_counter() - t0:.1f}s")
     stream = torch.cuda.Stream()  # default priority
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib << 18
+        a = torch.arange(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x torch.neg over {args.copy_probe_mib} MiB of int32 "
+            f"({n * 4} bytes read and written per dispatch)")
+        del a, b
     uncomp = batch.total_uncompressed
     algo = uncomp + batch.total_compressed
     forms = [f for f in args.forms.split(",") if f]
@@ -193,8 +268,11 @@ def main():
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(batch, args.format, threads)
         log(f"cpu baseline: {cpu}")
+    serving = serving_leg(args) if args.serve_rates else None
     traffic = None
     if args.traffic_file and os.path.exists(args.traffic_file):
+        # HBM bytes per launch of the tuned kernel from rocprofv3 FETCH_SIZE and
+        # WRITE_SIZE runs, scaled by a known-size copy (cudec_amd/profiling.py)
         with open(args.traffic_file) as f:
             traffic = json.load(f).get("bytes_per_launch")
     line = {
@@ -226,6 +304,9 @@ def main():
                          "algo_gbps": round(measured["plain"]["algo_gbps"], 2)}
         line["speedup_vs_plain"] = round(measured["plain"]["avg_kernel_ms"] / head["avg_kernel_ms"], 3)
     line["avg_kernel_ms"] = round(head["avg_kernel_ms"], 4)
+    if serving is not None:
+        line["serving"] = serving
+        line["p99_request_latency_ms"] = serving["rates"][-1]["p99_ms"]
     print(json.dumps(line), flush=True)
 
 

@@ -44,7 +44,8 @@ ABI_SYMBOLS = {
     "cudec_snappy.h": ["cudec_snappy_decompress_batch"],
     "cudec_mi355x.h": [
         "cudec_lz4_decompress_batch_plain", "cudec_snappy_decompress_batch_plain",
-        "cudec_mi355x_max_batch_chunks",
+        "cudec_mi355x_max_batch_chunks", "cudec_mi355x_digest_batch",
+        "cudec_mi355x_serve_lz4",
     ],
 }
 
@@ -91,6 +92,11 @@ def lib() -> ctypes.CDLL:
         L.cudec_lz4_decompress_stream.restype = ctypes.c_int
         L.cudec_lz4f_decompress.argtypes = [_vp, _sz, _vp, _sz, ctypes.POINTER(_sz)]
         L.cudec_lz4f_decompress.restype = ctypes.c_int
+        L.cudec_mi355x_digest_batch.argtypes = [_vp, _vp, _vp, _sz, _vp, _vp]
+        L.cudec_mi355x_digest_batch.restype = ctypes.c_int
+        L.cudec_mi355x_serve_lz4.argtypes = [_vp, _vp, _vp, _sz, _vp, _sz, _sz, ctypes.c_int,
+                                             _vp, _vp, _vp, _vp]
+        L.cudec_mi355x_serve_lz4.restype = ctypes.c_int
         _lib = L
     return _lib
 

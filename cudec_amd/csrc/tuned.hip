@@ -36,6 +36,8 @@
  * ladder reading global memory, with wide global-to-global copies.
  *
  * Snappy blocks run the plain form's body on the same persistent queue. */
+#include <mutex>
+
 #include "common.h"
 #include "ladder.h"
 
@@ -653,8 +655,60 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-int g_blocks_per_cu = 0;
-int g_cus = 0;
+/* Launch geometry and a scratch pool, per device, set up on first use. The
+ * pool keeps freed scratch (release threshold at its maximum) so that a
+ * stream of launches does not return memory to the driver between them. */
+struct DeviceState {
+    int cus = 0;
+    int blocks_per_cu = 0;
+    hipMemPool_t pool = nullptr;
+    bool ready = false;
+};
+constexpr int kMaxDevices = 64;
+DeviceState g_devices[kMaxDevices];
+std::mutex g_devices_lock;
+
+hipError_t device_state(DeviceState** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        return hipErrorInvalidDevice;
+    }
+    std::lock_guard<std::mutex> hold(g_devices_lock);
+    DeviceState& d = g_devices[dev];
+    if (!d.ready) {
+        e = hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) {
+            return e;
+        }
+        int per = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, tuned_batch<kFormatLz4>, kBlock, 0);
+        if (e != hipSuccess) {
+            return e;
+        }
+        d.blocks_per_cu = per > 0 ? per : 1;
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) {
+                d.pool = pool;
+            } else {
+                (void)hipMemPoolDestroy(pool);
+            }
+        }
+        (void)hipGetLastError(); /* a pool that could not be made is not an error */
+        d.ready = true;
+    }
+    *out = &d;
+    return hipSuccess;
+}
 
 }  // namespace
 
@@ -666,26 +720,15 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
          * plain form, which has no such limit */
         return launch_plain(format, srcs, src_sizes, dsts, caps, n, results, stream);
     }
-    if (g_cus == 0) {
-        int dev = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e != hipSuccess) {
-            return e;
-        }
-        e = hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e != hipSuccess) {
-            return e;
-        }
-        int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, tuned_batch<kFormatLz4>, kBlock, 0);
-        if (e != hipSuccess) {
-            return e;
-        }
-        g_blocks_per_cu = per > 0 ? per : 1;
+    DeviceState* dev = nullptr;
+    hipError_t e = device_state(&dev);
+    if (e != hipSuccess) {
+        return e;
     }
     const size_t bytes = sizeof(Queue) + sizeof(uint32_t) * n;
     void* scratch = nullptr;
-    hipError_t e = hipMallocAsync(&scratch, bytes, stream);
+    e = dev->pool != nullptr ? hipMallocFromPoolAsync(&scratch, bytes, dev->pool, stream)
+                             : hipMallocAsync(&scratch, bytes, stream);
     if (e != hipSuccess) {
         return e;
     }
@@ -697,7 +740,7 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     hipLaunchKernelGGL(queue_count, dim3(nb), dim3(tb), 0, stream, src_sizes, n, q);
     hipLaunchKernelGGL(queue_scan, dim3(1), dim3(1), 0, stream, q);
     hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, src_sizes, n, q, order);
-    size_t grid = (size_t)g_cus * (size_t)g_blocks_per_cu;
+    size_t grid = (size_t)dev->cus * (size_t)dev->blocks_per_cu;
     const size_t need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
     if (grid > need) {
         grid = need;

@@ -11,6 +11,8 @@
 
 #include "cudec.h"
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -35,6 +37,70 @@ cudec_status cudec_snappy_decompress_batch_plain(
 /* The largest chunk_count the batch entries accept (restates
  * src/batch_limits.h:28 for wave64: INT32_MAX * 64). */
 size_t cudec_mi355x_max_batch_chunks(void);
+
+/* No counterpart in the reference: a 64-bit digest of each chunk's bytes,
+ * computed on the device, for checking decoded output in place.
+ * Chunk i covers d_ptrs[i][0, len_i), where len_i is d_results[i]
+ * .bytes_written when d_results is given (0 unless its status is OK),
+ * else d_sizes[i]. Definition: len * 0x9E3779B97F4A7C15 plus, over the
+ * 8-byte little-endian words w_j of the zero-padded bytes, the sum of
+ * splitmix64_finalise(w_j + j * 0xD6E8FEB86659FD93), all mod 2^64.
+ * Arrays are device memory; asynchronous on stream like the batch entry. */
+cudec_status cudec_mi355x_digest_batch(const void* const* d_ptrs, const size_t* d_sizes,
+                                       const cudec_chunk_result* d_results,
+                                       size_t chunk_count, uint64_t* d_digests,
+                                       cudec_stream_t stream);
+
+/* ---------------------------------------------------------------- serving
+ *
+ * No counterpart in the reference. A request loop over the tuned LZ4
+ * batch decoder, run in real time on the calling thread:
+ *
+ *  - request r arrives at h_requests[r].arrival_s seconds after the call
+ *    starts; its chunks are entries [first_chunk, first_chunk + chunk_count)
+ *    of the chunk tables, whose compressed bytes are already in device
+ *    memory (h_src_ptrs holds device addresses, in a host array);
+ *  - admission is first come, first served against an HBM budget: a
+ *    request is admitted once its output (the sum of its chunks'
+ *    capacities, each rounded up to 256 bytes) fits in the free part of a
+ *    device output pool of hbm_budget bytes; the pool is allocated once;
+ *  - continuous batching: every admitted request not yet launched goes into
+ *    the next launch, on any of max_inflight streams that is idle, while
+ *    earlier launches are still running; a request's output is released
+ *    back to the pool when its launch completes;
+ *  - per chunk, the result record lands in h_results, and with h_digests
+ *    given, the digest of its output (cudec_mi355x_digest_batch) in
+ *    h_digests; per request, h_latency_s gets completion time minus
+ *    scheduled arrival (completion = when the loop sees its launch done).
+ *
+ * Returns CUDEC_ERR_INVALID_ARGUMENT (synchronously, before any device
+ * call) for NULL arrays, request_count or chunk_count 0, a request range
+ * outside the chunk tables, arrivals out of order, a request larger than
+ * the budget, or max_inflight outside [1, 16]; CUDEC_ERR_CUDA for a device
+ * failure; else CUDEC_OK (per-chunk outcomes are in h_results). */
+typedef struct cudec_mi355x_request {
+    double arrival_s;
+    size_t first_chunk;
+    size_t chunk_count;
+} cudec_mi355x_request;
+
+typedef struct cudec_mi355x_serve_stats {
+    double elapsed_s;          /* first arrival to last completion */
+    uint64_t launches;         /* batch launches */
+    uint64_t max_inflight;     /* most launches in flight at once */
+    uint64_t max_batch_requests; /* most requests in one launch */
+    uint64_t peak_pool_bytes;  /* most pool bytes held at once */
+    uint64_t output_bytes;     /* sum of bytes_written */
+    uint64_t compressed_bytes; /* sum of compressed sizes */
+} cudec_mi355x_serve_stats;
+
+cudec_status cudec_mi355x_serve_lz4(const void* const* h_src_ptrs, const size_t* h_src_sizes,
+                                    const size_t* h_dst_caps, size_t chunk_count,
+                                    const cudec_mi355x_request* h_requests,
+                                    size_t request_count, size_t hbm_budget,
+                                    int max_inflight, cudec_chunk_result* h_results,
+                                    uint64_t* h_digests, double* h_latency_s,
+                                    cudec_mi355x_serve_stats* stats);
 
 #ifdef __cplusplus
 }

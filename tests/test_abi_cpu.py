@@ -232,3 +232,63 @@ def test_test_side_xxh32_is_pinned():
         hc = 14 if f[4] & 0x08 else 6     # a zero size is written as "unknown"
         assert (libs.xxh32(f[4:hc]) >> 8) & 0xFF == f[hc]
         assert int.from_bytes(f[-4:], "little") == libs.xxh32(d), n
+
+
+def test_serve_rejects_without_a_device_call():
+    """cudec_mi355x_serve_lz4's synchronous argument checks."""
+    from cudec_amd import serve
+    L = cudec_amd.lib()
+    src = np.array([4096, 8192], np.uint64)
+    sizes = np.array([10, 10], np.uint64)
+    caps = np.array([1000, 1000], np.uint64)
+    res = np.zeros(2, dtype=serve.RESULT_DTYPE)
+    lat = np.zeros(2)
+
+    def call(req, n=2, budget=1 << 20, inflight=2, s=src, z=sizes, c=caps):
+        req = np.ascontiguousarray(req, dtype=serve.REQUEST_DTYPE)
+        return L.cudec_mi355x_serve_lz4(s.ctypes.data if s is not None else None, z.ctypes.data,
+                                        c.ctypes.data, n, req.ctypes.data, len(req), budget,
+                                        inflight, res.ctypes.data, None, lat.ctypes.data, None)
+
+    def reqs(*rows):
+        r = np.zeros(len(rows), dtype=serve.REQUEST_DTYPE)
+        for i, (a, f, k) in enumerate(rows):
+            r[i] = (a, f, k)
+        return r
+
+    I = cudec_amd.CUDEC_ERR_INVALID_ARGUMENT
+    good = reqs((0.0, 0, 1), (0.1, 1, 1))
+    assert call(good, s=None) == I
+    assert call(good, n=0) == I
+    assert call(good, inflight=0) == I and call(good, inflight=17) == I
+    assert call(reqs((0.0, 0, 3))) == I                    # range past the tables
+    assert call(reqs((0.0, 2, 1))) == I
+    assert call(reqs((0.0, 0, 0))) == I                    # empty request
+    assert call(reqs((0.2, 0, 1), (0.1, 1, 1))) == I       # arrivals out of order
+    assert call(reqs((float("nan"), 0, 1))) == I
+    assert call(good, budget=1000) == I                    # 1000 rounds up to 1024 > budget
+    assert call(reqs((0.0, 0, 2)), budget=1024 + 1023) == I  # two chunks need 2048
+
+
+def test_host_digest_definition():
+    """The host digest against values worked out by hand from its definition
+    (splitmix64 finaliser over position-mixed words)."""
+    from cudec_amd import serve
+
+    def mix(x):
+        M = (1 << 64) - 1
+        x ^= x >> 30
+        x = (x * 0xBF58476D1CE4E5B9) & M
+        x ^= x >> 27
+        x = (x * 0x94D049BB133111EB) & M
+        x ^= x >> 31
+        return x
+
+    assert serve.host_digest(b"") == 0
+    assert serve.host_digest(b"\x01") == (0x9E3779B97F4A7C15 + mix(1)) & ((1 << 64) - 1)
+    nine = bytes(range(1, 10))
+    w0 = int.from_bytes(nine[:8], "little")
+    w1 = 9 + 0xD6E8FEB86659FD93
+    want = (9 * 0x9E3779B97F4A7C15 + mix(w0) + mix(w1)) & ((1 << 64) - 1)
+    assert serve.host_digest(nine) == want
+    assert serve.host_digest(b"ab") != serve.host_digest(b"ba")

@@ -111,7 +111,7 @@ def test_many_waves_and_one_shot():
     """More chunks than one wave holds (double-buffered staging on both
     slots), through the one-shot entry."""
     n = 40000
-    streams = [b"\x00", b"\x40abcd\x04\x00\x50wxyz!"] * (n // 2)
+    streams = [b"\x00", b"\x44abcd\x04\x00\x50wxyz!"] * (n // 2)
     caps = [1, 17] * (n // 2)
     bufs, ptrs, sizes = _host_inputs(streams)
     outs = np.full(sum(caps), POISON, dtype=np.uint8)
