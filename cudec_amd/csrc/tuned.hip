@@ -55,10 +55,11 @@ constexpr uint32_t kRingMask = kRing - 1;
 constexpr uint32_t kBatchMax = 2048;      /* output bytes of one batch */
 constexpr uint32_t kStop = 0xFFFFu;       /* link of an unresolvable candidate */
 constexpr uint32_t kLaneLit = 64;         /* literal runs copied by their lane */
-constexpr uint32_t kLaneMatch = 32;       /* matches copied by their lane */
+constexpr uint32_t kLaneMatch = 32;       /* matches copied by their lane (<= 64) */
 constexpr int kBins = 64;
 
 static_assert(kBatchMax + 32 <= kRing, "ring must hold a batch and a margin");
+static_assert(kLaneMatch <= 64, "a lane match copies at most four 16-byte blocks");
 static_assert(kWin + 2 + 269 + 3 <= kStage, "a window's last candidate fits the stage");
 
 struct alignas(16) WaveLds {
@@ -204,10 +205,137 @@ __device__ __forceinline__ void flush_ring(const WaveLds& w, uint8_t* dst,
 }
 
 /* One output byte of history: from the ring when its position is still
- * there, else from global memory (already flushed by this wave). */
+ * there, else from global memory (already flushed by this wave). The
+ * global read takes the aligned 4-byte word holding the byte: a different
+ * load width keeps the compiler from merging the two branches into one
+ * flat load, and an aligned word never leaves the page of the byte. */
 __device__ __forceinline__ uint8_t history_byte(const WaveLds& w, const uint8_t* dst,
                                                 uint64_t y, uint64_t ring_lo) {
-    return y >= ring_lo ? w.ring[ring_index((uintptr_t)dst, y)] : dst[y];
+    if (y >= ring_lo) {
+        return w.ring[ring_index((uintptr_t)dst, y)];
+    }
+    const uintptr_t a = (uintptr_t)(dst + y);
+    const uint32_t word = *reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+    return (uint8_t)(word >> ((a & 3) * 8));
+}
+
+/* Byte i (0..15) of a 16-byte register block. */
+__device__ __forceinline__ uint32_t block_byte(const uint4& v, uint32_t i) {
+    const uint32_t x = (i >> 2) == 0 ? v.x : (i >> 2) == 1 ? v.y : (i >> 2) == 2 ? v.z : v.w;
+    return (x >> ((i & 3) * 8)) & 0xFF;
+}
+
+/* Exactly n (0..16) bytes of v to ring position r, wrapping at the ring's
+ * end: 8/4/2/1-byte stores, so no byte past n is touched. */
+__device__ __forceinline__ void ring_put(WaveLds& w, uint32_t r, const uint4& v, uint32_t n) {
+    if (n == 0) {
+        return;
+    }
+    if (r + n <= kRing) {
+        uint8_t* p = &w.ring[r];
+        if (n == 16) {
+            __builtin_memcpy(p, &v, 16);
+            return;
+        }
+        uint64_t lo = ((uint64_t)v.y << 32) | v.x;
+        const uint64_t hi = ((uint64_t)v.w << 32) | v.z;
+        if (n & 8) {
+            __builtin_memcpy(p, &lo, 8);
+            p += 8;
+            lo = hi;
+        }
+        if (n & 4) {
+            const uint32_t q = (uint32_t)lo;
+            __builtin_memcpy(p, &q, 4);
+            p += 4;
+            lo >>= 32;
+        }
+        if (n & 2) {
+            const uint16_t q = (uint16_t)lo;
+            __builtin_memcpy(p, &q, 2);
+            p += 2;
+            lo >>= 16;
+        }
+        if (n & 1) {
+            *p = (uint8_t)lo;
+        }
+        return;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        w.ring[(r + i) & kRingMask] = (uint8_t)block_byte(v, i);
+    }
+}
+
+/* One lane copies a match of len (<= kLaneMatch) bytes at output position
+ * m with offset off into the ring; every source byte precedes m or is
+ * written earlier by this lane.
+ *  - the source period [m - off, m - off + min(len, off)) lies wholly in
+ *    flushed history: up to four 16-byte global loads, issued together, so
+ *    the whole wave pays one round trip instead of one per byte;
+ *  - it lies wholly in the ring with off >= 8: 8-byte ring steps;
+ *  - otherwise the period is read bytewise.
+ * Bytes past the period repeat it from the ring, 8 bytes a step from a
+ * distance d that is a multiple of off and at least 8. */
+__device__ __forceinline__ void lane_match_copy(WaveLds& w, const uint8_t* dst, uintptr_t da,
+                                                uint64_t m, uint32_t off, uint32_t len,
+                                                uint64_t hist_lo, uint64_t cap) {
+    const uint64_t y = m - off;
+    const uint32_t need = len < off ? len : off;
+    const uint32_t rd = ring_index(da, m);
+    uint32_t i = 0;
+    if (y + need <= hist_lo && y + ((need + 15) & ~15u) <= cap) {
+        uint4 h0, h1 = make_uint4(0, 0, 0, 0), h2 = make_uint4(0, 0, 0, 0),
+                  h3 = make_uint4(0, 0, 0, 0);
+        __builtin_memcpy(&h0, dst + y, 16);
+        if (need > 16) {
+            __builtin_memcpy(&h1, dst + y + 16, 16);
+        }
+        if (need > 32) {
+            __builtin_memcpy(&h2, dst + y + 32, 16);
+        }
+        if (need > 48) {
+            __builtin_memcpy(&h3, dst + y + 48, 16);
+        }
+        ring_put(w, rd, h0, need < 16 ? need : 16);
+        if (need > 16) {
+            ring_put(w, (rd + 16) & kRingMask, h1, need - 16 < 16 ? need - 16 : 16);
+        }
+        if (need > 32) {
+            ring_put(w, (rd + 32) & kRingMask, h2, need - 32 < 16 ? need - 32 : 16);
+        }
+        if (need > 48) {
+            ring_put(w, (rd + 48) & kRingMask, h3, need - 48);
+        }
+        i = need;
+    } else if (y >= hist_lo && rd >= off && rd + len <= kRing && off >= 8) {
+        for (; i + 8 <= len; i += 8) {
+            uint64_t v;
+            __builtin_memcpy(&v, &w.ring[rd - off + i], 8);
+            __builtin_memcpy(&w.ring[rd + i], &v, 8);
+        }
+    } else if (y < hist_lo) {
+        for (; i < need; i++) {
+            w.ring[(rd + i) & kRingMask] = history_byte(w, dst, y + i, hist_lo);
+        }
+    }
+    if (i < len) {
+        /* d for off = 1..7 is 8, 8, 9, 8, 10, 12, 14 */
+        const uint32_t d =
+            off >= 8 ? off : (uint32_t)((0x0E0C0A08090808ull >> (8 * (off - 1))) & 0xFF);
+        for (; i < len && i < d; i++) {
+            w.ring[(rd + i) & kRingMask] = w.ring[(rd + i - off) & kRingMask];
+        }
+        if (rd + len <= kRing) {
+            for (; i + 8 <= len; i += 8) {
+                uint64_t v;
+                __builtin_memcpy(&v, &w.ring[rd + i - d], 8);
+                __builtin_memcpy(&w.ring[rd + i], &v, 8);
+            }
+        }
+        for (; i < len; i++) {
+            w.ring[(rd + i) & kRingMask] = w.ring[(rd + i - off) & kRingMask];
+        }
+    }
 }
 
 /* --------------------------------------------------- global slow path */
@@ -437,12 +565,26 @@ __device__ __forceinline__ ChunkOut decode_lz4_chunk(WaveLds& w, const uint8_t* 
                 const uint64_t lit_at = d0;
                 const uint64_t m_at = d0 + c.lit_len;
                 const bool wide_lit = c.lit_len > kLaneLit;
-                /* phase 1: literals, 4 bytes at a time; the overrun of at
-                 * most 3 bytes lands in this member's own match (>= 4) or,
-                 * for the terminal literal, past the batch where no history
-                 * read looks */
+                /* phase 1: literals, 16 bytes a step while whole steps
+                 * fit, then 4 bytes a step; the overrun of at most 3 bytes
+                 * lands in this member's own match (>= 4) or, for the
+                 * terminal literal, past the batch where no history read
+                 * looks */
                 if (active && !wide_lit) {
-                    for (uint32_t i = 0; i < c.lit_len; i += 4) {
+                    uint32_t i0 = 0;
+                    for (; i0 + 16 <= c.lit_len; i0 += 16) {
+                        uint4 v;
+                        __builtin_memcpy(&v, cw + c.lit + i0, 16);
+                        const uint32_t r = ring_index(da, lit_at + i0);
+                        if (r <= kRing - 16) {
+                            __builtin_memcpy(&w.ring[r], &v, 16);
+                        } else {
+                            for (uint32_t b = 0; b < 16; b++) {
+                                w.ring[(r + b) & kRingMask] = (uint8_t)block_byte(v, b);
+                            }
+                        }
+                    }
+                    for (uint32_t i = i0; i < c.lit_len; i += 4) {
                         uint32_t v;
                         __builtin_memcpy(&v, cw + c.lit + i, 4);
                         const uint32_t r = ring_index(da, lit_at + i);
@@ -464,10 +606,7 @@ __device__ __forceinline__ ChunkOut decode_lz4_chunk(WaveLds& w, const uint8_t* 
                 const bool lane_match = full && !wide_lit && c.match_len <= kLaneMatch &&
                                         independent;
                 if (lane_match) {
-                    for (uint32_t i = 0; i < c.match_len; i++) {
-                        const uint64_t x = m_at + i;
-                        w.ring[ring_index(da, x)] = history_byte(w, dst, x - c.offset, hist_lo);
-                    }
+                    lane_match_copy(w, dst, da, m_at, c.offset, c.match_len, hist_lo, cap);
                 }
                 wave_order();
                 /* phase 3: the rest, in stream order, whole wave */
@@ -595,7 +734,7 @@ __global__ void queue_fill(const size_t* sizes, size_t n, Queue* q, uint32_t* or
 }
 
 template <int kFormat>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
     tuned_batch(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
                 const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
                 const uint32_t* order) {

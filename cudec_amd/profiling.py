@@ -102,6 +102,40 @@ def traffic(fetch_csv: str, write_csv: str, kernel: str, copy: str, copy_bytes: 
             "scale_fetch": sf, "scale_write": sw}
 
 
+def traffic_markdown(t: dict, commands: list[str], source: str, algo_read: int,
+                     algo_write: int, title: str) -> str:
+    """The committed summary of a traffic measurement. It records the
+    sha256 of the kernel source measured, so a reader (bench.py) can tell
+    whether the figure still describes the current kernel."""
+    import hashlib
+    with open(source, "rb") as f:
+        digest = hashlib.sha256(f.read()).hexdigest()
+    cmd = "\n".join(f"    {c}" for c in commands)
+    return f"""# {title}
+
+Commands (repo root, MI355X), one counter per pass:
+
+{cmd}
+
+Calibration: the copy probe (`torch.neg` over 1 GiB of int32, so
+{t['copy_bytes']:,} B read and written per dispatch, {t['copy_dispatches'][0]} dispatches) reads
+FETCH_SIZE = {t['copy_fetch_kib']:.0f} KiB and WRITE_SIZE = {t['copy_write_kib']:.0f} KiB per dispatch. The scale
+factors are therefore {t['scale_fetch']:.4f} (fetch) and {t['scale_write']:.4f} (write).
+
+| per launch of `{t['kernel']}` ({t['dispatches'][0]} dispatches) | raw KiB | scaled bytes | algorithmic bytes |
+|---|---|---|---|
+| FETCH_SIZE | {t['fetch_kib_raw']:.0f} | {t['fetch_bytes']:,} | {algo_read:,} (compressed) |
+| WRITE_SIZE | {t['write_kib_raw']:.0f} | {t['write_bytes']:,} | {algo_write:,} (decoded) |
+| total | | {t['bytes_per_launch']:,} | {algo_read + algo_write:,} |
+
+Machine-readable lines (bench.py reports the figure as roofline.traffic
+only while the kernel source still has this hash):
+
+    bytes per launch: {t['bytes_per_launch']}
+    tuned.hip sha256: {digest}
+"""
+
+
 def main(argv=None) -> int:
     p = argparse.ArgumentParser(prog="python -m cudec_amd.profiling")
     sub = p.add_subparsers(dest="cmd", required=True)
@@ -117,6 +151,12 @@ def main(argv=None) -> int:
     t.add_argument("--copy", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
     t.add_argument("-o", "--out", required=True)
+    t.add_argument("--md", default=None, help="also write the committed markdown summary here")
+    t.add_argument("--source", default="cudec_amd/csrc/tuned.hip")
+    t.add_argument("--algo-read", type=int, default=0)
+    t.add_argument("--algo-write", type=int, default=0)
+    t.add_argument("--title", default="HBM traffic (rocprofv3 PMC)")
+    t.add_argument("--command", action="append", default=[])
     a = p.parse_args(argv)
     if a.cmd == "stats":
         md = stats_markdown(a.csv, a.command, a.title)
@@ -129,6 +169,10 @@ def main(argv=None) -> int:
         res = traffic(a.fetch, a.write, a.kernel, a.copy, a.copy_bytes)
         with open(a.out, "w") as f:
             json.dump(res, f, indent=1)
+        if a.md:
+            with open(a.md, "w") as f:
+                f.write(traffic_markdown(res, a.command, a.source, a.algo_read, a.algo_write,
+                                         a.title))
         print(json.dumps(res))
     return 0
 
