@@ -54,6 +54,9 @@ def parse_args():
                    help="before timing, run a plain elementwise copy (torch.neg over int32) "
                         "of this many MiB three times: the known-size kernel that HBM "
                         "counters are scaled by")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the default LZ4 pipelining threshold instead of tuning it "
+                        "on the batch before timing")
     p.add_argument("--serve-rates", default="2000,8000,32000",
                    help="request rates (per second) of the serving leg; empty to skip it")
     p.add_argument("--serve-seconds", type=float, default=0.5,
@@ -276,6 +279,21 @@ def main():
     default_config = (args.workload == "uneven" and args.format == "lz4" and args.chunks is None
                       and args.seed == 0x5EED and forms[0] == "tuned")
     measured = {}
+    tuning = None
+    if args.format == "lz4" and "tuned" in forms and not args.no_autotune:
+        import cudec_amd
+        t0 = time.perf_counter()
+        st, best, sweep = cudec_amd.autotune_lz4(
+            db.src_ptrs.data_ptr(), db.src_sizes.data_ptr(), db.dst_ptrs.data_ptr(),
+            db.dst_caps.data_ptr(), db.n, db.d_results.data_ptr(), stream.cuda_stream)
+        if st != 0:
+            raise SystemExit(f"[bench] autotune failed: {st}")
+        tuning = {"knob": "LZ4 chunks with cost (compressed + capacity) >= threshold decode "
+                          "on two pipelined waves, cheaper ones on one",
+                  "best_threshold_bytes": best,
+                  "sweep_ms": {("all" if c == 0 else "none" if c >= 2**63 else str(c)):
+                               round(ms, 3) for c, ms in sweep}}
+        log(f"autotune ({time.perf_counter() - t0:.1f}s): best threshold {best}; {tuning['sweep_ms']}")
     for form in forms:
         wall, kms = time_form(db, args.format, form, args.steps, args.warmup, stream, orig_dev,
                               batch.sizes)
@@ -331,6 +349,8 @@ def main():
                          "algo_gbps": round(measured["plain"]["algo_gbps"], 2)}
         line["speedup_vs_plain"] = round(measured["plain"]["avg_kernel_ms"] / head["avg_kernel_ms"], 3)
     line["avg_kernel_ms"] = round(head["avg_kernel_ms"], 4)
+    if tuning is not None:
+        line["autotune"] = tuning
     if serving is not None:
         line["serving"] = serving
         line["p99_request_latency_ms"] = serving["rates"][-1]["p99_ms"]

@@ -44,8 +44,9 @@ ABI_SYMBOLS = {
     "cudec_snappy.h": ["cudec_snappy_decompress_batch"],
     "cudec_mi355x.h": [
         "cudec_lz4_decompress_batch_plain", "cudec_snappy_decompress_batch_plain",
-        "cudec_mi355x_max_batch_chunks", "cudec_mi355x_digest_batch",
-        "cudec_mi355x_serve_lz4",
+        "cudec_mi355x_max_batch_chunks",
+        "cudec_mi355x_set_lz4_pipeline_threshold", "cudec_mi355x_lz4_pipeline_threshold",
+        "cudec_mi355x_autotune_lz4", "cudec_mi355x_digest_batch", "cudec_mi355x_serve_lz4",
     ],
 }
 
@@ -97,6 +98,13 @@ def lib() -> ctypes.CDLL:
         L.cudec_mi355x_serve_lz4.argtypes = [_vp, _vp, _vp, _sz, _vp, _sz, _sz, ctypes.c_int,
                                              _vp, _vp, _vp, _vp]
         L.cudec_mi355x_serve_lz4.restype = ctypes.c_int
+        L.cudec_mi355x_set_lz4_pipeline_threshold.argtypes = [_sz]
+        L.cudec_mi355x_set_lz4_pipeline_threshold.restype = ctypes.c_int
+        L.cudec_mi355x_lz4_pipeline_threshold.argtypes = [ctypes.POINTER(_sz)]
+        L.cudec_mi355x_lz4_pipeline_threshold.restype = ctypes.c_int
+        L.cudec_mi355x_autotune_lz4.argtypes = [_vp, _vp, _vp, _vp, _sz, _vp, _vp,
+                                                ctypes.POINTER(_sz), _vp, _vp]
+        L.cudec_mi355x_autotune_lz4.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -187,3 +195,31 @@ def cudec_lz4f_decompress(frame: bytes, capacity: int):
     st = lib().cudec_lz4f_decompress(ctypes.cast(src, _vp), len(frame), ctypes.cast(out, _vp),
                                      capacity, ctypes.byref(written))
     return st, (out.raw[:written.value] if st == CUDEC_OK else b"")
+
+
+# ------------------------------------------------------------------ tuning
+
+TUNE_CANDIDATES = 9
+
+
+def set_lz4_pipeline_threshold(min_cost: int) -> int:
+    return lib().cudec_mi355x_set_lz4_pipeline_threshold(min_cost)
+
+
+def lz4_pipeline_threshold():
+    """(status, min_cost) of the current device."""
+    v = _sz(0)
+    st = lib().cudec_mi355x_lz4_pipeline_threshold(ctypes.byref(v))
+    return st, v.value
+
+
+def autotune_lz4(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities, chunk_count,
+                 d_results, stream=None):
+    """Runs cudec_mi355x_autotune_lz4: (status, best_min_cost, [(candidate, ms)])."""
+    best = _sz(0)
+    cands = (_sz * TUNE_CANDIDATES)()
+    ms = (ctypes.c_float * TUNE_CANDIDATES)()
+    st = lib().cudec_mi355x_autotune_lz4(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
+                                         chunk_count, d_results, stream, ctypes.byref(best),
+                                         cands, ms)
+    return st, best.value, list(zip(list(cands), list(ms)))

@@ -38,7 +38,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}",
               f"-I{CSRC}", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 HIP_SOURCES = ["abi.hip", "plain.hip", "tuned.hip", "digest.hip"]
-HOST_SOURCES = ["stream.cpp", "frame.cpp", "serve.cpp"]
+HOST_SOURCES = ["stream.cpp", "frame.cpp", "serve.cpp", "tune.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -105,6 +105,24 @@ cudec_status cudec_snappy_decompress_batch_plain(
                   stream);
 }
 
+cudec_status cudec_mi355x_set_lz4_pipeline_threshold(size_t min_cost) {
+    (void)hipGetLastError();
+    return set_lz4_pipe_min_cost(min_cost) == hipSuccess ? CUDEC_OK : CUDEC_ERR_CUDA;
+}
+
+cudec_status cudec_mi355x_lz4_pipeline_threshold(size_t* min_cost) {
+    if (min_cost == nullptr) {
+        return CUDEC_ERR_INVALID_ARGUMENT;
+    }
+    (void)hipGetLastError();
+    uint64_t v = 0;
+    if (get_lz4_pipe_min_cost(&v) != hipSuccess) {
+        return CUDEC_ERR_CUDA;
+    }
+    *min_cost = (size_t)v;
+    return CUDEC_OK;
+}
+
 size_t cudec_mi355x_max_batch_chunks(void) { return kMaxBatchChunks; }
 
 }  // extern "C"

@@ -36,6 +36,8 @@
  * ladder reading global memory, with wide global-to-global copies.
  *
  * Snappy blocks run the plain form's body on the same persistent queue. */
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 
 #include "common.h"
@@ -56,7 +58,10 @@ constexpr uint32_t kBatchMax = 2048;      /* output bytes of one batch */
 constexpr uint32_t kStop = 0xFFFFu;       /* link of an unresolvable candidate */
 constexpr uint32_t kLaneLit = 64;         /* literal runs copied by their lane */
 constexpr uint32_t kLaneMatch = 32;       /* matches copied by their lane (<= 64) */
-constexpr int kBins = 64;
+constexpr int kBins = 256;
+/* Default: chunks costing at least 512 KiB (compressed size + capacity) run
+ * on two waves. */
+constexpr uint64_t kDefaultPipeMinCost = 512u << 10;
 
 static_assert(kBatchMax + 32 <= kRing, "ring must hold a batch and a margin");
 static_assert(kLaneMatch <= 64, "a lane match copies at most four 16-byte blocks");
@@ -88,9 +93,9 @@ __device__ __forceinline__ uint32_t ring_index(uintptr_t dst_addr, uint64_t pos)
  * chunk, so none leaves the chunk's pages); later blocks are zero-filled.
  * Bytes past src_size are never trusted: every member is validated against
  * src_size before anything it read is used. */
-__device__ __forceinline__ uint32_t stage_window(WaveLds& w, const uint8_t* src,
-                                                 uint64_t src_size, uint64_t sp,
-                                                 uint32_t lane) {
+__device__ __forceinline__ uint32_t stage_into(uint8_t* stage, const uint8_t* src,
+                                               uint64_t src_size, uint64_t sp,
+                                               uint32_t lane) {
     const uintptr_t a = (uintptr_t)(src + sp);
     const uintptr_t base = a & ~(uintptr_t)15;
     const uintptr_t lo = (uintptr_t)src;
@@ -102,7 +107,7 @@ __device__ __forceinline__ uint32_t stage_window(WaveLds& w, const uint8_t* src,
         if (g < end && g + 16 > lo) {
             v = *reinterpret_cast<const uint4*>(g);
         }
-        *reinterpret_cast<uint4*>(&w.stage[16u * blk]) = v;
+        *reinterpret_cast<uint4*>(&stage[16u * blk]) = v;
     }
     wave_order();
     return (uint32_t)(a - base);
@@ -391,6 +396,370 @@ __device__ __forceinline__ void wave_match_global(uint8_t* dst, uint64_t m, uint
     wave_order();
 }
 
+/* ------------------------------------------------- group execution */
+
+/* The members of one window that a group executes, as the copy phase needs
+ * them: per lane, the member's literal start (window-relative), literal
+ * and match lengths, offset and output offset from the group start; and,
+ * uniform, how many members run in full, whether member `take` adds its
+ * terminal literal only, and the group's output length. */
+struct Group {
+    uint32_t lit, lit_len, match_len, offset, drel;
+    uint32_t take;
+    bool terminal;
+    uint32_t end_rel;
+};
+
+/* Executes group g of the window staged at cw into the ring at output
+ * positions [dp, dp + g.end_rel), flushes whole 16-byte blocks (all of them
+ * when the chunk is done) and advances dp. */
+__device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, const Group& g,
+                                              uint8_t* dst, uint64_t cap, uint64_t& dp,
+                                              uint64_t& flushed, uint64_t ring_lo, bool done,
+                                              uint32_t lane) {
+    const uintptr_t da = (uintptr_t)dst;
+    const uint32_t take = g.take;
+    const bool full = lane < take;
+    const bool active = full || (g.terminal && lane == take);
+    const uint32_t lit = g.lit, lit_len = g.lit_len, match_len = g.match_len,
+                   offset = g.offset, drel = g.drel;
+    const uint64_t D0 = dp;
+    const uint64_t D1 = dp + g.end_rel;
+    const uint64_t ring_floor = D1 + 16 > kRing ? D1 + 16 - kRing : 0;
+    const uint64_t hist_lo = ring_lo > ring_floor ? ring_lo : ring_floor;
+    const uint64_t lit_at = D0 + drel;
+    const uint64_t m_at = lit_at + lit_len;
+    const bool wide_lit = lit_len > kLaneLit;
+    /* phase 1: literals, 16 bytes a step while whole steps fit, then 4
+     * bytes a step; the overrun of at most 3 bytes lands in this member's
+     * own match (>= 4) or, for the terminal literal, past the group where
+     * no history read looks */
+    if (active && !wide_lit) {
+        uint32_t i0 = 0;
+        for (; i0 + 16 <= lit_len; i0 += 16) {
+            uint4 v;
+            __builtin_memcpy(&v, cw + lit + i0, 16);
+            const uint32_t r = ring_index(da, lit_at + i0);
+            if (r <= kRing - 16) {
+                __builtin_memcpy(&w.ring[r], &v, 16);
+            } else {
+                for (uint32_t b = 0; b < 16; b++) {
+                    w.ring[(r + b) & kRingMask] = (uint8_t)block_byte(v, b);
+                }
+            }
+        }
+        for (uint32_t i = i0; i < lit_len; i += 4) {
+            uint32_t v;
+            __builtin_memcpy(&v, cw + lit + i, 4);
+            const uint32_t r = ring_index(da, lit_at + i);
+            if (r <= kRing - 4) {
+                __builtin_memcpy(&w.ring[r], &v, 4);
+            } else {
+                for (uint32_t b = 0; b < 4; b++) {
+                    w.ring[(r + b) & kRingMask] = (uint8_t)(v >> (8 * b));
+                }
+            }
+        }
+    }
+    wave_order();
+    /* phase 2: matches that read only history or their own literals, by
+     * their own lane */
+    const uint64_t y = m_at - offset;
+    const uint64_t need = match_len < offset ? match_len : offset;
+    const bool independent = y + need <= D0 || y >= lit_at;
+    const bool lane_match = full && !wide_lit && match_len <= kLaneMatch && independent;
+    if (lane_match) {
+        lane_match_copy(w, dst, da, m_at, offset, match_len, hist_lo, cap);
+    }
+    wave_order();
+    /* phase 3: the rest, in stream order, whole wave */
+    uint64_t rest = __ballot(active && (wide_lit || (full && !lane_match)));
+    while (rest != 0) {
+        const uint32_t j = __builtin_ctzll(rest);
+        rest &= rest - 1;
+        const uint32_t jl = __builtin_amdgcn_readlane(lit_len, j);
+        const uint32_t jlit = __builtin_amdgcn_readlane(lit, j);
+        const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
+        const uint64_t jd0 = D0 + jd;
+        if (jl > kLaneLit) {
+            for (uint32_t i = lane; i < jl; i += kWave) {
+                w.ring[ring_index(da, jd0 + i)] = cw[jlit + i];
+            }
+        }
+        wave_order();
+        if (j < take) {
+            const uint32_t jm = __builtin_amdgcn_readlane(match_len, j);
+            const uint32_t joff = __builtin_amdgcn_readlane(offset, j);
+            const bool jdone = __builtin_amdgcn_readlane((uint32_t)lane_match, j) != 0;
+            if (!jdone) {
+                const uint64_t jm0 = jd0 + jl;
+                const uint64_t jfrom = jm0 - joff;
+                for (uint32_t i = lane; i < jm; i += kWave) {
+                    const uint64_t yy = jfrom + (i < joff ? i : i % joff);
+                    w.ring[ring_index(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
+                }
+            }
+        }
+        wave_order();
+    }
+    dp = D1;
+    const uintptr_t a_end = done ? da + dp : ((da + dp) & ~(uintptr_t)15);
+    const uint64_t to = a_end > da + flushed ? (uint64_t)(a_end - da) : flushed;
+    if (to > flushed) {
+        flush_ring(w, dst, flushed, to, lane);
+        flushed = to;
+    }
+}
+
+/* ------------------------------------------------- window parse */
+
+/* What parsing one window decided. kGroup: execute `g` (staged at `delta`
+ * in the stage buffer) and advance the source by `consumed`. kSlow: one
+ * sequence the window cannot hold, by the scalar ladder (`q`, with the
+ * cursor after it in sp/dp). kFail: the chunk ends with `status`. */
+enum ParseKind : uint32_t { kParseGroup = 0, kParseSlow = 1, kParseFail = 2 };
+
+struct Parsed {
+    uint32_t kind;
+    int status;
+    bool done;        /* the chunk ends after this group / sequence */
+    uint32_t delta;
+    Group g;
+    Lz4Step q;
+};
+
+/* Stages the window at sp into `stage`, speculates, walks and resolves it
+ * (src/lz4_block.h:95-258 for every member, in parallel) and advances
+ * sp and dp past what it decided. Reads the source only. */
+__device__ __forceinline__ Parsed parse_window(uint8_t* stage, const uint8_t* src,
+                                               uint64_t src_size, uint64_t cap, uint64_t& sp,
+                                               uint64_t& dp, uint32_t lane) {
+    Parsed out;
+    out.kind = kParseFail;
+    out.status = kOk;
+    out.done = false;
+    out.delta = stage_into(stage, src, src_size, sp, lane);
+    const uint8_t* cw = stage + out.delta;
+
+    /* speculate: the link of each candidate position */
+    uint32_t link[kCand];
+#pragma unroll
+    for (uint32_t k = 0; k < kCand; k++) {
+        const Cand c = parse_candidate(cw, k * kWave + lane);
+        link[k] = c.complex ? kStop : c.next;
+    }
+
+    /* walk the true chain */
+    uint32_t pos = 0, count = 0;
+    uint32_t member = 0; /* lane j: position of member j */
+    bool stopped = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kCand; k++) {
+        while (!stopped && pos < (k + 1) * kWave && count < kWave) {
+            const uint32_t nx = __builtin_amdgcn_readlane(link[k], pos - k * kWave);
+            if (nx == kStop) {
+                stopped = true;
+                break;
+            }
+            member = lane == count ? pos : member;
+            count++;
+            pos = nx;
+        }
+    }
+    count = uniform32(count);
+    pos = uniform32(pos);
+
+    bool slow = count == 0;
+    if (!slow) {
+        /* resolve members: the exact ladder for each member's sequence */
+        const bool is_member = lane < count;
+        const Cand c = parse_candidate(cw, is_member ? member : 0);
+        const uint32_t out_len = is_member ? c.lit_len + c.match_len : 0;
+        const uint32_t incl = wave_inclusive_sum(out_len, lane);
+        const uint32_t drel = incl - out_len;
+        const uint64_t d0 = dp + drel; /* absolute destination */
+        int ev = kOk;
+        bool terminal = false;
+        if (is_member) {
+            uint64_t s = sp + member;  /* absolute source position */
+            if (s >= src_size) {
+                ev = kCorrupt;
+            } else {
+                s++;
+                if ((c.tok >> 4) == 15) {
+                    if (s + 15 >= src_size) {
+                        ev = kCorrupt;
+                    } else if (15 > cap - d0) {
+                        ev = kTooSmall;
+                    } else {
+                        s++;
+                        if (s + 15 > src_size) {
+                            ev = kCorrupt;
+                        }
+                    }
+                }
+                const uint64_t L = c.lit_len;
+                if (ev == kOk) {
+                    if (L > src_size - s) {
+                        ev = kCorrupt;
+                    } else if (L > cap - d0) {
+                        ev = kTooSmall;
+                    } else if (cap - d0 - L < kLz4SlackDst || src_size - s - L < kLz4SlackSrc) {
+                        if (s + L != src_size) {
+                            ev = kCorrupt;
+                        } else if (cap == 0 && c.tok != 0) {
+                            ev = kCorrupt;
+                        } else {
+                            terminal = true;
+                        }
+                    } else {
+                        const uint64_t dl = d0 + L;
+                        uint64_t M = (c.tok & 15) + 4;
+                        s += L + 2;
+                        if (c.offset == 0 || c.offset > dl) {
+                            ev = kCorrupt;
+                        } else {
+                            if ((c.tok & 15) == 15) {
+                                if (s >= src_size) {
+                                    ev = kCorrupt;
+                                } else if (M > cap - dl) {
+                                    ev = kTooSmall;
+                                } else {
+                                    s++;
+                                    M = c.match_len;
+                                    if (s + kLz4MatchMargin > src_size) {
+                                        ev = kCorrupt;
+                                    }
+                                }
+                            }
+                            if (ev == kOk) {
+                                if (M > cap - dl) {
+                                    ev = kTooSmall;
+                                } else if (M > cap - dl - kLz4LastLiterals) {
+                                    ev = kCorrupt;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        const uint32_t mine = terminal ? c.lit_len : out_len;
+        const bool cut = is_member && drel + mine > kBatchMax;
+        const bool flag = is_member && (cut || ev != kOk || terminal);
+        const uint64_t flags = __ballot(flag);
+        uint32_t take = count; /* members executed in full */
+        bool take_terminal = false;
+        int status = kOk;
+        if (flags != 0) {
+            const uint32_t j = __builtin_ctzll(flags);
+            take = j;
+            const bool j_cut = __builtin_amdgcn_readlane((uint32_t)cut, j) != 0;
+            const int j_ev = (int)__builtin_amdgcn_readlane((uint32_t)ev, j);
+            if (j_cut) {
+                slow = j == 0;
+            } else if (j_ev != kOk) {
+                status = j_ev;
+            } else {
+                take_terminal = true;
+            }
+        }
+        take = uniform32(take);
+        take_terminal = uniform32(take_terminal) != 0;
+        slow = uniform32(slow) != 0;
+        status = (int)uniform32((uint32_t)status);
+        if (status != kOk) {
+            out.status = status;
+            return out;
+        }
+        if (!slow && (take != 0 || take_terminal)) {
+            const uint32_t last = take_terminal ? take : take - 1;
+            out.kind = kParseGroup;
+            out.g.lit = c.lit;
+            out.g.lit_len = c.lit_len;
+            out.g.match_len = c.match_len;
+            out.g.offset = c.offset;
+            out.g.drel = drel;
+            out.g.take = take;
+            out.g.terminal = take_terminal;
+            out.g.end_rel = uniform32(__builtin_amdgcn_readlane(incl, last) -
+                                      (take_terminal ? __builtin_amdgcn_readlane(c.match_len, last)
+                                                     : 0));
+            const uint32_t consumed =
+                take_terminal ? (uint32_t)(src_size - sp)
+                : take == count ? pos
+                                : __builtin_amdgcn_readlane(member, take);
+            sp = uniform64(sp + consumed);
+            dp = uniform64(dp + out.g.end_rel);
+            out.done = take_terminal;
+            return out;
+        }
+    }
+    /* one sequence by the exact scalar ladder */
+    Lz4Cursor cur{src, src_size, cap, sp, dp};
+    bool last = false;
+    const int st = (int)uniform32((uint32_t)lz4_next(cur, out.q, last));
+    if (st != kOk) {
+        out.status = st;
+        return out;
+    }
+    out.kind = kParseSlow;
+    out.done = uniform32(last) != 0;
+    sp = uniform64(cur.sp);
+    dp = uniform64(cur.dp);
+    return out;
+}
+
+/* Executes a sequence the window could not hold: everything in the ring
+ * goes out first, then the literal and the match run global to global. */
+__device__ __forceinline__ void execute_slow(WaveLds& w, const uint8_t* src, uint8_t* dst,
+                                             const Lz4Step& q, uint64_t dp_after, uint64_t& dp,
+                                             uint64_t& flushed, uint64_t& ring_lo,
+                                             uint32_t lane) {
+    flush_ring(w, dst, flushed, dp, lane);
+    wave_copy_global(dst + q.lit_dst, src + q.lit_src, q.lit_len, lane);
+    if (q.match_len != 0) {
+        wave_match_global(dst, q.match_dst, q.offset, q.match_len, lane);
+    }
+    dp = dp_after;
+    flushed = dp;
+    ring_lo = dp;
+}
+
+/* ------------------------------------------------- two-wave pipeline */
+
+/* The window record the parsing wave hands the copying wave: one Parsed,
+ * laid out per lane in LDS. */
+struct alignas(16) PipeRec {
+    uint16_t lit[kWave];
+    uint16_t lit_len[kWave];
+    uint16_t match_len[kWave];
+    uint16_t offset[kWave];
+    uint16_t drel[kWave];
+    uint32_t kind;
+    int32_t status;
+    uint32_t done;
+    uint32_t delta;
+    uint32_t take;
+    uint32_t terminal;
+    uint32_t end_rel;
+    uint32_t pad;
+    uint64_t lit_src, lit_dst, lit_len_s, match_dst, offset_s, match_len_s, dp_after;
+};
+
+struct alignas(16) PipeLds {
+    WaveLds w;                  /* ring (copier) and stage buffer 0 */
+    WaveLds w2;                 /* the second wave's own, once the block splits */
+    uint8_t stage1[kStageBuf];  /* stage buffer 1 */
+    PipeRec rec[2];
+    uint32_t chunk_idx;
+    uint32_t fin;               /* step whose record ends the chunk */
+    uint32_t pad[2];
+};
+
+static_assert(kLaneLit < 65536 && kBatchMax < 65536 && kStageBuf < 65536,
+              "window records hold 16-bit fields");
+
 /* ------------------------------------------------------------- LZ4 */
 
 struct ChunkOut {
@@ -398,290 +767,32 @@ struct ChunkOut {
     uint64_t produced;
 };
 
+/* One wave per chunk: parse a window, then execute it, in turn. */
 __device__ __forceinline__ ChunkOut decode_lz4_chunk(WaveLds& w, const uint8_t* src,
-                                                  uint64_t src_size, uint8_t* dst,
-                                                  uint64_t cap, uint32_t lane) {
-    uint64_t sp = 0, dp = 0;
-    uint64_t flushed = 0;  /* output positions below this are in global memory */
-    uint64_t ring_lo = 0;  /* output positions at or above this are in the ring */
-    int status = kOk;
-    bool done = false;
+                                                     uint64_t src_size, uint8_t* dst,
+                                                     uint64_t cap, uint32_t lane) {
+    uint64_t sp = 0, pdp = 0, dp = 0, flushed = 0, ring_lo = 0;
     uint64_t fuel = src_size == UINT64_MAX ? UINT64_MAX : src_size + 2;
-    const uintptr_t da = (uintptr_t)dst;
-
-    while (status == kOk && !done) {
+    int status = kOk;
+    for (;;) {
         if (fuel-- == 0) {
             status = kCorrupt;
             break;
         }
-        const uint32_t delta = stage_window(w, src, src_size, sp, lane);
-        const uint8_t* cw = w.stage + delta;
-
-        /* 2. speculate: the link of each candidate position */
-        uint32_t link[kCand];
-#pragma unroll
-        for (uint32_t k = 0; k < kCand; k++) {
-            const Cand c = parse_candidate(cw, k * kWave + lane);
-            link[k] = c.complex ? kStop : c.next;
+        const Parsed pr = parse_window(w.stage, src, src_size, cap, sp, pdp, lane);
+        if (pr.kind == kParseFail) {
+            status = pr.status;
+            break;
         }
-
-        /* 3. walk the true chain */
-        uint32_t pos = 0, count = 0;
-        uint32_t member = 0; /* lane j: position of member j */
-        bool stopped = false;
-#pragma unroll
-        for (uint32_t k = 0; k < kCand; k++) {
-            while (!stopped && pos < (k + 1) * kWave && count < kWave) {
-                const uint32_t nx = __builtin_amdgcn_readlane(link[k], pos - k * kWave);
-                if (nx == kStop) {
-                    stopped = true;
-                    break;
-                }
-                member = lane == count ? pos : member;
-                count++;
-                pos = nx;
-            }
+        if (pr.kind == kParseGroup) {
+            execute_group(w, w.stage + pr.delta, pr.g, dst, cap, dp, flushed, ring_lo, pr.done,
+                          lane);
+        } else {
+            execute_slow(w, src, dst, pr.q, pdp, dp, flushed, ring_lo, lane);
         }
-        count = uniform32(count);
-        pos = uniform32(pos);
-
-        bool slow = uniform32(count == 0) != 0;
-        uint64_t batch_out = 0;
-        if (!slow) {
-            /* 4. resolve members */
-            const bool is_member = lane < count;
-            const Cand c = parse_candidate(cw, is_member ? member : 0);
-            const uint32_t out_len = is_member ? c.lit_len + c.match_len : 0;
-            const uint32_t incl = wave_inclusive_sum(out_len, lane);
-            const uint32_t drel = incl - out_len;
-            /* the exact ladder for this member's sequence */
-            const uint64_t s0 = sp + member; /* absolute source position */
-            const uint64_t d0 = dp + drel;   /* absolute destination */
-            int ev = kOk;
-            bool terminal = false;
-            if (is_member) {
-                uint64_t s = s0;
-                if (s >= src_size) {
-                    ev = kCorrupt;
-                } else {
-                    s++;
-                    const uint32_t ln = c.tok >> 4;
-                    if (ln == 15) {
-                        if (s + 15 >= src_size) {
-                            ev = kCorrupt;
-                        } else if (15 > cap - d0) {
-                            ev = kTooSmall;
-                        } else {
-                            s++;
-                            if (s + 15 > src_size) {
-                                ev = kCorrupt;
-                            }
-                        }
-                    }
-                    const uint64_t L = c.lit_len;
-                    if (ev == kOk) {
-                        if (L > src_size - s) {
-                            ev = kCorrupt;
-                        } else if (L > cap - d0) {
-                            ev = kTooSmall;
-                        } else if (cap - d0 - L < kLz4SlackDst ||
-                                   src_size - s - L < kLz4SlackSrc) {
-                            if (s + L != src_size) {
-                                ev = kCorrupt;
-                            } else if (cap == 0 && c.tok != 0) {
-                                ev = kCorrupt;
-                            } else {
-                                terminal = true;
-                            }
-                        } else {
-                            const uint64_t dl = d0 + L;
-                            uint64_t M = (c.tok & 15) + 4;
-                            s += L + 2;
-                            if (c.offset == 0 || c.offset > dl) {
-                                ev = kCorrupt;
-                            } else {
-                                if ((c.tok & 15) == 15) {
-                                    if (s >= src_size) {
-                                        ev = kCorrupt;
-                                    } else if (M > cap - dl) {
-                                        ev = kTooSmall;
-                                    } else {
-                                        s++;
-                                        M = c.match_len;
-                                        if (s + kLz4MatchMargin > src_size) {
-                                            ev = kCorrupt;
-                                        }
-                                    }
-                                }
-                                if (ev == kOk) {
-                                    if (M > cap - dl) {
-                                        ev = kTooSmall;
-                                    } else if (M > cap - dl - kLz4LastLiterals) {
-                                        ev = kCorrupt;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            const uint32_t mine = terminal ? c.lit_len : out_len;
-            const bool cut = is_member && drel + mine > kBatchMax;
-            const bool flag = is_member && (cut || ev != kOk || terminal);
-            const uint64_t flags = __ballot(flag);
-            uint32_t take = count; /* members executed in full */
-            bool take_terminal = false;
-            if (flags != 0) {
-                const uint32_t j = __builtin_ctzll(flags);
-                take = j;
-                const bool j_cut = __builtin_amdgcn_readlane((uint32_t)cut, j) != 0;
-                const int j_ev = (int)__builtin_amdgcn_readlane((uint32_t)ev, j);
-                if (j_cut) {
-                    slow = j == 0;
-                } else if (j_ev != kOk) {
-                    status = j_ev;
-                } else {
-                    take_terminal = true;
-                }
-            }
-            take = uniform32(take);
-            take_terminal = uniform32(take_terminal) != 0;
-            slow = uniform32(slow) != 0;
-            status = (int)uniform32((uint32_t)status);
-            if (status == kOk && !slow) {
-                /* 5. execute members [0, take) and the terminal's literal */
-                const bool full = lane < take;
-                const bool lit_only = take_terminal && lane == take;
-                const bool active = full || lit_only;
-                const uint32_t last = take_terminal ? take : take - 1;
-                const uint32_t end_rel =
-                    __builtin_amdgcn_readlane(incl, last) -
-                    (take_terminal ? __builtin_amdgcn_readlane(c.match_len, last) : 0);
-                batch_out = end_rel;
-                const uint64_t D0 = dp;
-                const uint64_t D1 = dp + end_rel;
-                const uint64_t ring_floor = D1 + 16 > kRing ? D1 + 16 - kRing : 0;
-                const uint64_t hist_lo = ring_lo > ring_floor ? ring_lo : ring_floor;
-                const uint64_t lit_at = d0;
-                const uint64_t m_at = d0 + c.lit_len;
-                const bool wide_lit = c.lit_len > kLaneLit;
-                /* phase 1: literals, 16 bytes a step while whole steps
-                 * fit, then 4 bytes a step; the overrun of at most 3 bytes
-                 * lands in this member's own match (>= 4) or, for the
-                 * terminal literal, past the batch where no history read
-                 * looks */
-                if (active && !wide_lit) {
-                    uint32_t i0 = 0;
-                    for (; i0 + 16 <= c.lit_len; i0 += 16) {
-                        uint4 v;
-                        __builtin_memcpy(&v, cw + c.lit + i0, 16);
-                        const uint32_t r = ring_index(da, lit_at + i0);
-                        if (r <= kRing - 16) {
-                            __builtin_memcpy(&w.ring[r], &v, 16);
-                        } else {
-                            for (uint32_t b = 0; b < 16; b++) {
-                                w.ring[(r + b) & kRingMask] = (uint8_t)block_byte(v, b);
-                            }
-                        }
-                    }
-                    for (uint32_t i = i0; i < c.lit_len; i += 4) {
-                        uint32_t v;
-                        __builtin_memcpy(&v, cw + c.lit + i, 4);
-                        const uint32_t r = ring_index(da, lit_at + i);
-                        if (r <= kRing - 4) {
-                            __builtin_memcpy(&w.ring[r], &v, 4);
-                        } else {
-                            for (uint32_t b = 0; b < 4; b++) {
-                                w.ring[(r + b) & kRingMask] = (uint8_t)(v >> (8 * b));
-                            }
-                        }
-                    }
-                }
-                wave_order();
-                /* phase 2: matches that read only history or their own
-                 * literals, by their own lane */
-                const uint64_t y = m_at - c.offset;
-                const uint64_t need = c.match_len < c.offset ? c.match_len : c.offset;
-                const bool independent = y + need <= D0 || y >= lit_at;
-                const bool lane_match = full && !wide_lit && c.match_len <= kLaneMatch &&
-                                        independent;
-                if (lane_match) {
-                    lane_match_copy(w, dst, da, m_at, c.offset, c.match_len, hist_lo, cap);
-                }
-                wave_order();
-                /* phase 3: the rest, in stream order, whole wave */
-                uint64_t rest = __ballot(active && (wide_lit || (full && !lane_match)));
-                while (rest != 0) {
-                    const uint32_t j = __builtin_ctzll(rest);
-                    rest &= rest - 1;
-                    const uint32_t jl = __builtin_amdgcn_readlane(c.lit_len, j);
-                    const uint32_t jlit = __builtin_amdgcn_readlane(c.lit, j);
-                    const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
-                    const uint64_t jd0 = dp + jd;
-                    if (jl > kLaneLit) {
-                        for (uint32_t i = lane; i < jl; i += kWave) {
-                            w.ring[ring_index(da, jd0 + i)] = cw[jlit + i];
-                        }
-                    }
-                    wave_order();
-                    if (j < take) {
-                        const uint32_t jm = __builtin_amdgcn_readlane(c.match_len, j);
-                        const uint32_t joff = __builtin_amdgcn_readlane(c.offset, j);
-                        const bool jdone = __builtin_amdgcn_readlane((uint32_t)lane_match, j) != 0;
-                        if (!jdone) {
-                            const uint64_t jm0 = jd0 + jl;
-                            const uint64_t jfrom = jm0 - joff;
-                            for (uint32_t i = lane; i < jm; i += kWave) {
-                                const uint64_t yy = jfrom + (i < joff ? i : i % joff);
-                                w.ring[ring_index(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
-                            }
-                        }
-                    }
-                    wave_order();
-                }
-                /* advance */
-                const uint32_t consumed =
-                    take_terminal ? (uint32_t)(src_size - sp)
-                    : take == count ? pos
-                                    : __builtin_amdgcn_readlane(member, take);
-                sp = uniform64(sp + consumed);
-                dp = uniform64(D1);
-                done = take_terminal;
-                /* 6. flush */
-                /* flush complete 16-byte blocks (everything when done);
-                 * computed in address space so an unaligned start cannot
-                 * underflow the target */
-                const uintptr_t a_end = done ? da + dp : ((da + dp) & ~(uintptr_t)15);
-                const uint64_t to = a_end > da + flushed ? (uint64_t)(a_end - da) : flushed;
-                if (to > flushed) {
-                    flush_ring(w, dst, flushed, to, lane);
-                    flushed = to;
-                }
-                slow = take == 0 && !take_terminal;
-            }
+        if (pr.done) {
+            break;
         }
-        if (status == kOk && !done && slow) {
-            /* one sequence by the exact scalar ladder, global memory */
-            flush_ring(w, dst, flushed, dp, lane);
-            flushed = dp;
-            Lz4Cursor cur{src, src_size, cap, sp, dp};
-            Lz4Step q;
-            bool last = false;
-            status = (int)uniform32((uint32_t)lz4_next(cur, q, last));
-            last = uniform32(last) != 0;
-            if (status == kOk) {
-                wave_copy_global(dst + q.lit_dst, src + q.lit_src, q.lit_len, lane);
-                if (q.match_len != 0) {
-                    wave_match_global(dst, q.match_dst, q.offset, q.match_len, lane);
-                }
-                sp = uniform64(cur.sp);
-                dp = uniform64(cur.dp);
-                flushed = dp;
-                ring_lo = dp;
-                done = last;
-            }
-        }
-        (void)batch_out;
     }
     ChunkOut r;
     r.status = status;
@@ -702,16 +813,28 @@ __global__ void queue_reset(Queue* q) {
     }
 }
 
-/* Bin of a chunk: larger compressed size, smaller bin (decoded first). */
-__device__ __forceinline__ uint32_t size_bin(uint64_t size) {
-    const uint32_t lg = size == 0 ? 0 : 64u - (uint32_t)__builtin_clzll(size);
-    return (kBins - 1) - (lg < kBins - 1 ? lg : kBins - 1);
+/* Bin of a chunk by its decode cost, compressed size plus capacity: a
+ * chunk's time follows the bytes it produces (a 4 MiB run of short-offset
+ * matches compresses to a few hundred KiB yet decodes as long as 4 MiB of
+ * literals and far matches). Four bins per octave; larger cost, smaller
+ * bin, decoded first. */
+__device__ __forceinline__ uint32_t cost_bin(uint64_t src_size, uint64_t cap) {
+    const uint64_t lim = 1ull << 62;
+    const uint64_t a = src_size < lim ? src_size : lim;
+    const uint64_t b = cap < lim ? cap : lim;
+    const uint64_t cost = a + b;
+    if (cost < 4) {
+        return kBins - 1 - (uint32_t)cost;
+    }
+    const uint32_t lg = 63u - (uint32_t)__builtin_clzll(cost);   /* 2..62 */
+    const uint32_t key = lg * 4 + (uint32_t)((cost >> (lg - 2)) & 3);
+    return kBins - 1 - key;
 }
 
-__global__ void queue_count(const size_t* sizes, size_t n, Queue* q) {
+__global__ void queue_count(const size_t* sizes, const size_t* caps, size_t n, Queue* q) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        atomicAdd(&q->bin_count[size_bin(sizes[i])], 1u);
+        atomicAdd(&q->bin_count[cost_bin(sizes[i], caps[i])], 1u);
     }
 }
 
@@ -725,10 +848,11 @@ __global__ void queue_scan(Queue* q) {
     }
 }
 
-__global__ void queue_fill(const size_t* sizes, size_t n, Queue* q, uint32_t* order) {
+__global__ void queue_fill(const size_t* sizes, const size_t* caps, size_t n, Queue* q,
+                           uint32_t* order) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        const uint32_t b = size_bin(sizes[i]);
+        const uint32_t b = cost_bin(sizes[i], caps[i]);
         order[q->bin_base[b] + atomicAdd(&q->bin_cursor[b], 1u)] = (uint32_t)i;
     }
 }
@@ -794,16 +918,188 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     }
 }
 
+constexpr int kPipeBlock = 2 * kWave;
+constexpr uint32_t kNoFin = 0xFFFFFFFFu;
+
+/* LZ4, two waves per chunk. Wave 0 parses window i into record i % 2 (and
+ * stage buffer i % 2) while wave 1 executes record i - 1; one barrier
+ * separates the steps. Parsing reads only the source and decides every
+ * status; execution only writes the ring and the destination, so the
+ * chunk's critical path is the longer of the two per window rather than
+ * their sum. Control flow around the barriers is uniform for the block:
+ * both waves read the chunk index and the final step from LDS after a
+ * barrier, and a barrier opens every chunk so that no control word is
+ * rewritten while the other wave may still read it. */
+__global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu(4)))
+    piped_lz4(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
+              const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
+              const uint32_t* order, uint32_t pipe_bin) {
+    __shared__ PipeLds L;
+    const uint32_t lane = __lane_id();
+    const bool parser = uniform32(threadIdx.x / kWave) == 0;
+    uint32_t split_idx = 0xFFFFFFFFu; /* first chunk decoded one wave per chunk */
+    for (;;) {
+        /* both waves are done reading the previous chunk's control words
+         * (chunk_idx, fin) before the parser writes the next chunk's */
+        __syncthreads();
+        if (parser) {
+            /* every lane takes part; only lane 0 adds (see tuned_batch) */
+            const uint32_t mine = atomicAdd(&q->next, lane == 0 ? 1u : 0u);
+            L.chunk_idx = uniform32(mine);
+            L.fin = kNoFin;
+        }
+        __syncthreads();
+        const uint32_t idx = uniform32(L.chunk_idx);
+        if (idx >= n) {
+            break;
+        }
+        const uint64_t chunk = order[idx];
+        const uint8_t* src = uniform_ptr(static_cast<const uint8_t*>(srcs[chunk]));
+        uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[chunk]));
+        const uint64_t src_size = uniform64(src_sizes[chunk]);
+        const uint64_t cap = uniform64(caps[chunk]);
+        if (cost_bin(src_size, cap) > pipe_bin) {
+            /* the queue runs largest cost first: every chunk from here on
+             * is cheaper too, so the block splits for good */
+            split_idx = idx;
+            break;
+        }
+        uint64_t sp = 0, dp = 0, flushed = 0, ring_lo = 0;
+        uint64_t fuel = src_size == UINT64_MAX ? UINT64_MAX : src_size + 2;
+        int status = kOk;
+        bool pdone = false;
+        for (uint32_t step = 0;; step++) {
+            const uint32_t b = step & 1;
+            if (parser) {
+                if (!pdone) {
+                    PipeRec& r = L.rec[b];
+                    Parsed pr;
+                    if (fuel-- == 0) {
+                        pr.kind = kParseFail;
+                        pr.status = kCorrupt;
+                        pr.done = false;
+                        pr.delta = 0;
+                    } else {
+                        pr = parse_window(b ? L.stage1 : L.w.stage, src, src_size, cap, sp, dp,
+                                          lane);
+                    }
+                    r.kind = pr.kind;
+                    r.status = pr.status;
+                    r.done = pr.done ? 1u : 0u;
+                    if (pr.kind == kParseGroup) {
+                        r.lit[lane] = (uint16_t)pr.g.lit;
+                        r.lit_len[lane] = (uint16_t)pr.g.lit_len;
+                        r.match_len[lane] = (uint16_t)pr.g.match_len;
+                        r.offset[lane] = (uint16_t)pr.g.offset;
+                        r.drel[lane] = (uint16_t)pr.g.drel;
+                        r.delta = pr.delta;
+                        r.take = pr.g.take;
+                        r.terminal = pr.g.terminal ? 1u : 0u;
+                        r.end_rel = pr.g.end_rel;
+                    } else if (pr.kind == kParseSlow) {
+                        r.lit_src = pr.q.lit_src;
+                        r.lit_dst = pr.q.lit_dst;
+                        r.lit_len_s = pr.q.lit_len;
+                        r.match_dst = pr.q.match_dst;
+                        r.offset_s = pr.q.offset;
+                        r.match_len_s = pr.q.match_len;
+                        r.dp_after = dp;
+                    }
+                    pdone = pr.kind == kParseFail || pr.done;
+                    if (pdone) {
+                        L.fin = step;
+                    }
+                }
+            } else if (step > 0 && status == kOk) {
+                const PipeRec& r = L.rec[b ^ 1];
+                const uint32_t kind = uniform32(r.kind);
+                if (kind == kParseFail) {
+                    status = (int)uniform32((uint32_t)r.status);
+                } else if (kind == kParseGroup) {
+                    Group g;
+                    g.lit = r.lit[lane];
+                    g.lit_len = r.lit_len[lane];
+                    g.match_len = r.match_len[lane];
+                    g.offset = r.offset[lane];
+                    g.drel = r.drel[lane];
+                    g.take = uniform32(r.take);
+                    g.terminal = uniform32(r.terminal) != 0;
+                    g.end_rel = uniform32(r.end_rel);
+                    const uint8_t* cw = (b ? L.w.stage : L.stage1) + uniform32(r.delta);
+                    execute_group(L.w, cw, g, dst, cap, dp, flushed, ring_lo,
+                                  uniform32(r.done) != 0, lane);
+                } else {
+                    Lz4Step s;
+                    s.lit_src = uniform64(r.lit_src);
+                    s.lit_dst = uniform64(r.lit_dst);
+                    s.lit_len = uniform64(r.lit_len_s);
+                    s.match_dst = uniform64(r.match_dst);
+                    s.offset = (uint32_t)uniform64(r.offset_s);
+                    s.match_len = uniform64(r.match_len_s);
+                    execute_slow(L.w, src, dst, s, uniform64(r.dp_after), dp, flushed, ring_lo,
+                                 lane);
+                }
+            }
+            __syncthreads();
+            const uint32_t fin = uniform32(L.fin);
+            if (fin != kNoFin && step == fin + 1) {
+                break;
+            }
+        }
+        if (!parser) {
+            write_result(results, chunk, status, dp);
+        }
+    }
+    if (split_idx == 0xFFFFFFFFu) {
+        return;
+    }
+    /* one wave per chunk from here: the parser decodes the chunk in hand,
+     * the other wave takes its own from the queue; no barrier follows */
+    WaveLds& w = parser ? L.w : L.w2;
+    uint32_t idx = parser ? split_idx : uniform32(atomicAdd(&q->next, lane == 0 ? 1u : 0u));
+    while (idx < n) {
+        const uint64_t chunk = order[idx];
+        const uint8_t* src = uniform_ptr(static_cast<const uint8_t*>(srcs[chunk]));
+        uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[chunk]));
+        const uint64_t src_size = uniform64(src_sizes[chunk]);
+        const uint64_t cap = uniform64(caps[chunk]);
+        const ChunkOut o = decode_lz4_chunk(w, src, src_size, dst, cap, lane);
+        write_result(results, chunk, (int)uniform32((uint32_t)o.status), uniform64(o.produced));
+        idx = uniform32(atomicAdd(&q->next, lane == 0 ? 1u : 0u));
+    }
+}
+
 /* Launch geometry and a scratch pool, per device, set up on first use. The
  * pool keeps freed scratch (release threshold at its maximum) so that a
  * stream of launches does not return memory to the driver between them. */
 struct DeviceState {
     int cus = 0;
-    int blocks_per_cu = 0;
+    int blocks_per_cu = 0;       /* tuned_batch (one wave per chunk) */
+    int pipe_blocks_per_cu = 0;  /* piped_lz4 (two waves per chunk) */
+    uint64_t lz4_pipe_min_cost = kDefaultPipeMinCost; /* two waves at or above this cost */
     hipMemPool_t pool = nullptr;
     bool ready = false;
 };
 constexpr int kMaxDevices = 64;
+
+/* The cost bin at or below which (costlier chunks) the pipelined form runs a
+ * chunk on two waves; UINT32_MAX-free mapping of a byte threshold. */
+uint32_t pipe_bin_for(uint64_t min_cost) {
+    if (min_cost == 0) {
+        return kBins - 1;   /* every chunk */
+    }
+    if (min_cost == UINT64_MAX) {
+        return 0xFFFFFFFFu; /* none: the first chunk splits the block */
+    }
+    /* the bin of a chunk whose cost is exactly min_cost (host mirror of
+     * cost_bin); costlier chunks have bins at or below it */
+    const uint64_t c = min_cost;
+    if (c < 4) {
+        return (uint32_t)(kBins - 1 - c);
+    }
+    const uint32_t lg = 63u - (uint32_t)__builtin_clzll(c);
+    return (uint32_t)(kBins - 1 - (lg * 4 + (uint32_t)((c >> (lg - 2)) & 3)));
+}
 DeviceState g_devices[kMaxDevices];
 std::mutex g_devices_lock;
 
@@ -829,6 +1125,11 @@ hipError_t device_state(DeviceState** out) {
             return e;
         }
         d.blocks_per_cu = per > 0 ? per : 1;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_lz4, kPipeBlock, 0);
+        if (e != hipSuccess) {
+            return e;
+        }
+        d.pipe_blocks_per_cu = per > 0 ? per : 1;
         hipMemPoolProps props = {};
         props.allocType = hipMemAllocationTypePinned;
         props.location.type = hipMemLocationTypeDevice;
@@ -851,6 +1152,28 @@ hipError_t device_state(DeviceState** out) {
 
 }  // namespace
 
+hipError_t set_lz4_pipe_min_cost(uint64_t min_cost) {
+    DeviceState* dev = nullptr;
+    const hipError_t e = device_state(&dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    std::lock_guard<std::mutex> hold(g_devices_lock);
+    dev->lz4_pipe_min_cost = min_cost;
+    return hipSuccess;
+}
+
+hipError_t get_lz4_pipe_min_cost(uint64_t* min_cost) {
+    DeviceState* dev = nullptr;
+    const hipError_t e = device_state(&dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    std::lock_guard<std::mutex> hold(g_devices_lock);
+    *min_cost = dev->lz4_pipe_min_cost;
+    return hipSuccess;
+}
+
 hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_sizes,
                         void* const* dsts, const size_t* caps, size_t n,
                         cudec_chunk_result* results, hipStream_t stream) {
@@ -864,6 +1187,11 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     if (e != hipSuccess) {
         return e;
     }
+    uint64_t min_cost = 0;
+    {
+        std::lock_guard<std::mutex> hold(g_devices_lock);
+        min_cost = dev->lz4_pipe_min_cost;
+    }
     const size_t bytes = sizeof(Queue) + sizeof(uint32_t) * n;
     void* scratch = nullptr;
     e = dev->pool != nullptr ? hipMallocFromPoolAsync(&scratch, bytes, dev->pool, stream)
@@ -876,20 +1204,31 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     const unsigned tb = 256;
     const unsigned nb = (unsigned)((n + tb - 1) / tb);
     hipLaunchKernelGGL(queue_reset, dim3(1), dim3(kBins), 0, stream, q);
-    hipLaunchKernelGGL(queue_count, dim3(nb), dim3(tb), 0, stream, src_sizes, n, q);
+    hipLaunchKernelGGL(queue_count, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q);
     hipLaunchKernelGGL(queue_scan, dim3(1), dim3(1), 0, stream, q);
-    hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, src_sizes, n, q, order);
-    size_t grid = (size_t)dev->cus * (size_t)dev->blocks_per_cu;
-    const size_t need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (grid > need) {
-        grid = need;
-    }
-    if (format == kFormatSnappy) {
-        hipLaunchKernelGGL(tuned_batch<kFormatSnappy>, dim3((unsigned)grid), dim3(kBlock), 0,
-                           stream, srcs, src_sizes, dsts, caps, n, results, q, order);
+    hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q, order);
+    const uint32_t pipe_bin = pipe_bin_for(min_cost);
+    if (format == kFormatLz4 && pipe_bin != 0xFFFFFFFFu) {
+        /* one chunk per two-wave block at a time */
+        size_t grid = (size_t)dev->cus * (size_t)dev->pipe_blocks_per_cu;
+        if (grid > n) {
+            grid = n;
+        }
+        hipLaunchKernelGGL(piped_lz4, dim3((unsigned)grid), dim3(kPipeBlock), 0, stream, srcs,
+                           src_sizes, dsts, caps, n, results, q, order, pipe_bin);
     } else {
-        hipLaunchKernelGGL(tuned_batch<kFormatLz4>, dim3((unsigned)grid), dim3(kBlock), 0,
-                           stream, srcs, src_sizes, dsts, caps, n, results, q, order);
+        size_t grid = (size_t)dev->cus * (size_t)dev->blocks_per_cu;
+        const size_t need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
+        if (grid > need) {
+            grid = need;
+        }
+        if (format == kFormatSnappy) {
+            hipLaunchKernelGGL(tuned_batch<kFormatSnappy>, dim3((unsigned)grid), dim3(kBlock), 0,
+                               stream, srcs, src_sizes, dsts, caps, n, results, q, order);
+        } else {
+            hipLaunchKernelGGL(tuned_batch<kFormatLz4>, dim3((unsigned)grid), dim3(kBlock), 0,
+                               stream, srcs, src_sizes, dsts, caps, n, results, q, order);
+        }
     }
     e = hipGetLastError();
     const hipError_t f = hipFreeAsync(scratch, stream);

@@ -38,6 +38,32 @@ cudec_status cudec_snappy_decompress_batch_plain(
  * src/batch_limits.h:28 for wave64: INT32_MAX * 64). */
 size_t cudec_mi355x_max_batch_chunks(void);
 
+/* No counterpart in the reference: launch tuning of the LZ4 batch entry
+ * (cudec_lz4_decompress_batch) on the current device. Inside one launch the
+ * chunks run costliest first (cost = compressed size + capacity); a chunk
+ * whose cost is at least `min_cost` bytes decodes on two waves (one
+ * parses the next window while the other executes the current one), a
+ * cheaper one on a single wave. 0 puts every chunk on two waves, SIZE_MAX
+ * none. Output bytes and results never depend on this setting. Default:
+ * 512 KiB. Settings are per device and process-wide. */
+cudec_status cudec_mi355x_set_lz4_pipeline_threshold(size_t min_cost);
+cudec_status cudec_mi355x_lz4_pipeline_threshold(size_t* min_cost);
+
+/* No counterpart in the reference: the autotuner of that threshold. It
+ * decodes the caller's batch (arguments as cudec_lz4_decompress_batch) once
+ * per candidate threshold after a warmup, synchronously, keeps the fastest
+ * for the current device and returns it in *h_best_min_cost. With them
+ * non-NULL, h_candidates and h_ms (CUDEC_MI355X_TUNE_CANDIDATES entries)
+ * receive each candidate and its best event-timed launch in milliseconds.
+ * The batch's outputs and results are those of the last run. */
+#define CUDEC_MI355X_TUNE_CANDIDATES 9
+cudec_status cudec_mi355x_autotune_lz4(const void* const* d_src_ptrs,
+                                       const size_t* d_src_sizes, void* const* d_dst_ptrs,
+                                       const size_t* d_dst_capacities, size_t chunk_count,
+                                       cudec_chunk_result* d_results, cudec_stream_t stream,
+                                       size_t* h_best_min_cost, size_t* h_candidates,
+                                       float* h_ms);
+
 /* No counterpart in the reference: a 64-bit digest of each chunk's bytes,
  * computed on the device, for checking decoded output in place.
  * Chunk i covers d_ptrs[i][0, len_i), where len_i is d_results[i]

@@ -292,3 +292,11 @@ def test_host_digest_definition():
     want = (9 * 0x9E3779B97F4A7C15 + mix(w0) + mix(w1)) & ((1 << 64) - 1)
     assert serve.host_digest(nine) == want
     assert serve.host_digest(b"ab") != serve.host_digest(b"ba")
+
+
+def test_tuning_entries_reject_without_a_device_call():
+    I = cudec_amd.CUDEC_ERR_INVALID_ARGUMENT
+    assert cudec_amd.autotune_lz4(16, 16, 16, 16, 1, 24, None)[0] == I     # misaligned results
+    assert cudec_amd.autotune_lz4(None, 16, 16, 16, 1, 16, None)[0] == I
+    assert cudec_amd.autotune_lz4(16, 16, 16, 16, 0, 16, None)[0] == I
+    assert cudec_amd.lib().cudec_mi355x_lz4_pipeline_threshold(None) == I
