@@ -102,6 +102,23 @@ def traffic(fetch_csv: str, write_csv: str, kernel: str, copy: str, copy_bytes: 
             "scale_fetch": sf, "scale_write": sw}
 
 
+def counters_markdown(path: str, kernel: str, command: str, title: str,
+                      per_unit: int = 0, unit_name: str = "") -> str:
+    """Average per dispatch of every counter collected for kernels whose
+    full name contains `kernel`, and, with per_unit, the count per unit."""
+    acc = read_counters(path)
+    names = sorted({c for (k, c) in acc if kernel in k})
+    rows = []
+    for c in names:
+        mean, n = _mean_for(acc, kernel, c)
+        extra = f" | {mean / per_unit:.2f}" if per_unit else ""
+        rows.append(f"| {c} | {n} | {mean:,.0f}{extra} |")
+    head = "| counter | dispatches | mean per dispatch |" + (f" per {unit_name} |" if per_unit else "")
+    sep = "|---|---|---|" + ("---|" if per_unit else "")
+    return "\n".join([f"# {title}", "", "Command (repo root, MI355X):", "", f"    {command}", "",
+                      head, sep] + rows) + "\n"
+
+
 def traffic_markdown(t: dict, commands: list[str], source: str, algo_read: int,
                      algo_write: int, title: str) -> str:
     """The committed summary of a traffic measurement. It records the
@@ -144,6 +161,14 @@ def main(argv=None) -> int:
     s.add_argument("--command", required=True)
     s.add_argument("--title", default="rocprofv3 kernel stats")
     s.add_argument("-o", "--out", default=None)
+    c = sub.add_parser("counters")
+    c.add_argument("csv")
+    c.add_argument("--kernel", required=True)
+    c.add_argument("--command", required=True)
+    c.add_argument("--title", default="rocprofv3 counters")
+    c.add_argument("--per-unit", type=int, default=0)
+    c.add_argument("--unit-name", default="")
+    c.add_argument("-o", "--out", default=None)
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
@@ -158,7 +183,14 @@ def main(argv=None) -> int:
     t.add_argument("--title", default="HBM traffic (rocprofv3 PMC)")
     t.add_argument("--command", action="append", default=[])
     a = p.parse_args(argv)
-    if a.cmd == "stats":
+    if a.cmd == "counters":
+        md = counters_markdown(a.csv, a.kernel, a.command, a.title, a.per_unit, a.unit_name)
+        if a.out:
+            with open(a.out, "w") as f:
+                f.write(md)
+        else:
+            sys.stdout.write(md)
+    elif a.cmd == "stats":
         md = stats_markdown(a.csv, a.command, a.title)
         if a.out:
             with open(a.out, "w") as f:
