@@ -57,7 +57,7 @@ constexpr uint32_t kRingMask = kRing - 1;
 constexpr uint32_t kBatchMax = 2048;      /* output bytes of one batch */
 constexpr uint32_t kStop = 0xFFFFu;       /* link of an unresolvable candidate */
 constexpr uint32_t kLaneLit = 64;         /* literal runs copied by their lane */
-constexpr uint32_t kLaneMatch = 32;       /* matches copied by their lane (<= 64) */
+constexpr uint32_t kLaneMatch = 64;       /* matches copied by their lane (<= 64) */
 constexpr int kBins = 256;
 /* Default: chunks costing at least 512 KiB (compressed size + capacity) run
  * on two waves. */
@@ -374,24 +374,78 @@ __device__ __forceinline__ void wave_copy_global(uint8_t* dst, const uint8_t* sr
     wave_order();
 }
 
-/* dst[m + i] = dst[m - off + (i mod off)] for i < len, whole wave, in
- * global memory; every source byte precedes m and is final. Rounds of at
- * most `off` bytes keep every read behind every write of the same round. */
-__device__ __forceinline__ void wave_match_global(uint8_t* dst, uint64_t m, uint64_t off,
-                                                  uint64_t len, uint32_t lane) {
-    if (off >= 16) {
-        /* whole periods: each round copies min(off, 1024) bytes */
-        const uint64_t round = off < 1024 ? off : 1024;
-        for (uint64_t r0 = 0; r0 < len; r0 += round) {
-            const uint64_t n = len - r0 < round ? len - r0 : round;
+/* Largest period staged in the ring by wave_match_global. */
+constexpr uint32_t kStagedPeriod = 1024;
+static_assert(kStagedPeriod + 16 <= kRing, "a staged period fits the ring");
+
+/* dst[m + i] = dst[m - off + (i mod off)] for i < len (cap bounds dst),
+ * whole wave, in global memory; every source byte precedes m and is final.
+ * The ring is free scratch here (everything in it is flushed):
+ *  - off >= len: one contiguous copy;
+ *  - off < kStagedPeriod: the period, extended by 16 bytes, is staged in
+ *    the ring once, then every lane stores 16-byte slices read from it at
+ *    (i mod off) -- one global round trip for the whole match, where a
+ *    byte-per-lane gather pays one per 64 bytes and whole-period rounds one
+ *    per period;
+ *  - otherwise: rounds of off (>= kStagedPeriod) bytes. */
+__device__ __forceinline__ void wave_match_global(WaveLds& w, uint8_t* dst, uint64_t m,
+                                                  uint64_t off, uint64_t len, uint64_t cap,
+                                                  uint32_t lane) {
+    if (off >= len) {
+        wave_copy_global(dst + m, dst + m - off, len, lane);
+        return;
+    }
+    if (off >= kStagedPeriod) {
+        for (uint64_t r0 = 0; r0 < len; r0 += off) {
+            const uint64_t n = len - r0 < off ? len - r0 : off;
             wave_copy_global(dst + m + r0, dst + m + r0 - off, n, lane);
         }
         return;
     }
     const uint32_t o = (uint32_t)off;
-    const uint64_t from = m - off;
-    for (uint64_t i = lane; i < len; i += kWave) {
-        dst[m + i] = dst[from + (uint32_t)(i % o)];
+    const uint64_t from = m - o;
+    /* the period into ring[0, o): 16-byte loads where the block stays in
+     * [0, cap); the bytes a block reads past the period are overwritten by
+     * the extension below */
+    for (uint32_t b = 16 * lane; b < o; b += 16 * kWave) {
+        uint4 v;
+        if (from + b + 16 <= cap) {
+            __builtin_memcpy(&v, dst + from + b, 16);
+        } else {
+            uint32_t q[4] = {0, 0, 0, 0};
+            for (uint32_t i = 0; i < 16 && b + i < o; i++) {
+                q[i >> 2] |= (uint32_t)dst[from + b + i] << ((i & 3) * 8);
+            }
+            v = make_uint4(q[0], q[1], q[2], q[3]);
+        }
+        __builtin_memcpy(&w.ring[b], &v, 16);
+    }
+    wave_order();
+    /* ring[o + t] = ring[t mod o] for t < 16 */
+    if (lane < 16) {
+        uint32_t k = lane;
+        while (k >= o) {
+            k -= o;
+        }
+        const uint8_t e = w.ring[k];
+        w.ring[o + lane] = e;
+    }
+    wave_order();
+    uint32_t r = (16 * lane) % o;
+    const uint32_t step = (16 * kWave) % o;
+    for (uint64_t b = 16 * lane; b < len; b += 16 * kWave) {
+        uint4 v;
+        __builtin_memcpy(&v, &w.ring[r], 16);
+        if (len - b >= 16) {
+            __builtin_memcpy(dst + m + b, &v, 16);
+        } else {
+            const uint32_t n = (uint32_t)(len - b);
+            for (uint32_t i = 0; i < n; i++) {
+                dst[m + b + i] = (uint8_t)block_byte(v, i);
+            }
+        }
+        r += step;
+        r = r >= o ? r - o : r;
     }
     wave_order();
 }
@@ -713,13 +767,14 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, const uint8_t* sr
 /* Executes a sequence the window could not hold: everything in the ring
  * goes out first, then the literal and the match run global to global. */
 __device__ __forceinline__ void execute_slow(WaveLds& w, const uint8_t* src, uint8_t* dst,
-                                             const Lz4Step& q, uint64_t dp_after, uint64_t& dp,
+                                             uint64_t cap, const Lz4Step& q, uint64_t dp_after,
+                                             uint64_t& dp,
                                              uint64_t& flushed, uint64_t& ring_lo,
                                              uint32_t lane) {
     flush_ring(w, dst, flushed, dp, lane);
     wave_copy_global(dst + q.lit_dst, src + q.lit_src, q.lit_len, lane);
     if (q.match_len != 0) {
-        wave_match_global(dst, q.match_dst, q.offset, q.match_len, lane);
+        wave_match_global(w, dst, q.match_dst, q.offset, q.match_len, cap, lane);
     }
     dp = dp_after;
     flushed = dp;
@@ -788,7 +843,7 @@ __device__ __forceinline__ ChunkOut decode_lz4_chunk(WaveLds& w, const uint8_t* 
             execute_group(w, w.stage + pr.delta, pr.g, dst, cap, dp, flushed, ring_lo, pr.done,
                           lane);
         } else {
-            execute_slow(w, src, dst, pr.q, pdp, dp, flushed, ring_lo, lane);
+            execute_slow(w, src, dst, cap, pr.q, pdp, dp, flushed, ring_lo, lane);
         }
         if (pr.done) {
             break;
@@ -900,7 +955,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
                     break;
                 }
                 if (e.is_copy) {
-                    wave_match_global(dst, e.to, e.offset, e.length, lane);
+                    wave_match_global(w, dst, e.to, e.offset, e.length, cap, lane);
                 } else {
                     wave_copy_global(dst + e.to, src + e.from, e.length, lane);
                 }
@@ -1036,8 +1091,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                     s.match_dst = uniform64(r.match_dst);
                     s.offset = (uint32_t)uniform64(r.offset_s);
                     s.match_len = uniform64(r.match_len_s);
-                    execute_slow(L.w, src, dst, s, uniform64(r.dp_after), dp, flushed, ring_lo,
-                                 lane);
+                    execute_slow(L.w, src, dst, cap, s, uniform64(r.dp_after), dp, flushed,
+                                 ring_lo, lane);
                 }
             }
             __syncthreads();

@@ -260,3 +260,26 @@ def test_forms_and_streams_deterministic(lz4_fixtures):
     res = db.results()
     assert np.array_equal(res, ref_res)
     assert ok_slots(db, res, db.output_arena()) == ref_slots
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_long_periodic_matches(form):
+    """Periodic data compresses to long matches (two or more extension
+    bytes) at the period's offset, ending at the chunk's last byte: the
+    wide copy paths for offsets below, at and above the staged-period limit
+    (1024), with capacity exactly the output size so no staged read may pass
+    it, plus one byte short (OUTPUT_TOO_SMALL)."""
+    rng = np.random.default_rng(0x9E41)
+    streams, caps, names = [], [], []
+    for period in (1, 2, 3, 5, 7, 8, 15, 16, 17, 31, 64, 100, 1000, 1023, 1024, 1025, 2000):
+        for n in (300, 1000, 5000, 70000):
+            if period * 2 > n:
+                continue
+            base = rng.integers(0, 256, period, dtype=np.uint8).tobytes()
+            data = (base * (n // period + 1))[:n]
+            s = libs.lz4_compress(data)
+            for cap in (n, n - 1):
+                streams.append(s)
+                caps.append(cap)
+                names.append(f"period{period}/n{n}/cap{cap}")
+    check_against_oracle("lz4", form, streams, caps, names)
