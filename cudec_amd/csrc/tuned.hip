@@ -1,41 +1,42 @@
-/* The tuned launch form of the LZ4 batch decoder for gfx950.
+/* The tuned launch form of the LZ4 and Snappy batch decoders for gfx950.
  *
- * Work distribution: a persistent grid (LDS-limited residency on every CU)
- * pulls chunk indices from a device queue in which the chunks are binned by
- * compressed size, largest bin first, so the longest decodes start earliest.
+ * Work distribution: four small kernels bin the chunks by decode cost
+ * (compressed size + capacity; four bins per octave) into a device queue,
+ * costliest first. A persistent grid of two-wave blocks then pulls chunks
+ * from it. A chunk at or above the tuned cost threshold runs on both waves
+ * of its block, as a software pipeline. Once the queue reaches cheaper
+ * chunks, a block splits into two independent one-wave decoders.
  *
- * Per chunk, one wave64 decodes in windows of kWin candidate token positions
+ * Per chunk, the stream is decoded in windows of kWin candidate positions
  * staged in LDS:
- *   1. stage   -- kStage compressed bytes from the window start are loaded as
- *                 aligned 16-byte blocks (a block holding any byte of the
- *                 chunk never leaves the chunk's pages) into LDS;
- *   2. speculate -- every lane parses 8 candidate positions as if a token
- *                 started there (literal length with one extension byte,
- *                 offset, match length with one extension byte) and records
- *                 where the next token would start;
- *   3. walk    -- a scalar walk follows the true chain from position 0
- *                 through those links (v_readlane), writing each member's
- *                 position into lane j: up to 64 sequences;
- *   4. resolve -- each member lane re-derives its sequence, a wavefront
- *                 prefix sum places its output, and the exact validation
- *                 ladder (src/lz4_block.h:95-258) runs per lane; the first
- *                 member in stream order that rejects, terminates, or would
- *                 overflow the batch ends the batch, so the chunk's status
- *                 is the sequential ladder's;
- *   5. execute -- output goes through a per-wave LDS ring: literals by
- *                 member lanes, independent short matches by member lanes,
- *                 the rest (long runs, matches reading this batch's output)
- *                 in stream order with the whole wave fanned out over the
- *                 bytes (closed-form gather dst[m + i] = dst[m - off +
- *                 (i mod off)]);
- *   6. flush   -- completed ring bytes leave as aligned 16-byte stores; only
- *                 the chunk's first and last partial blocks use byte stores,
- *                 so exactly bytes_written bytes are ever stored.
- * A sequence the window cannot hold (a length with two or more extension
- * bytes, or an output larger than one batch) is decoded by the exact scalar
- * ladder reading global memory, with wide global-to-global copies.
- *
- * Snappy blocks run the plain form's body on the same persistent queue. */
+ *   1. stage     -- compressed bytes from the window start are loaded as
+ *                   aligned 16-byte blocks (a block holding any byte of the
+ *                   chunk never leaves the chunk's pages);
+ *   2. speculate -- every lane parses 8 candidate positions as if a sequence
+ *                   (LZ4) or element (Snappy) started there, and records
+ *                   where the next one would start;
+ *   3. walk      -- a scalar walk follows the true chain from position 0
+ *                   through those links (v_readlane): up to 64 members, one
+ *                   per lane;
+ *   4. resolve   -- a wavefront prefix sum places each member's output and
+ *                   the exact validation ladder (src/lz4_block.h:95-258,
+ *                   src/snappy_block.h:288-435) runs per lane; the first
+ *                   member in stream order that rejects, ends the stream,
+ *                   or would overflow the group ends the group, so a chunk's
+ *                   status is the sequential ladder's;
+ *   5. execute   -- output goes through an LDS ring: literals by member
+ *                   lanes; matches whose sources are final by member lanes
+ *                   (batched 16-byte history loads); the rest in stream
+ *                   order with the wave fanned out over the bytes;
+ *   6. flush     -- completed ring bytes leave as aligned 16-byte stores;
+ *                   only a chunk's first and last partial blocks use byte
+ *                   stores, so exactly bytes_written bytes are stored.
+ * Steps 1-4 read only the source and decide every status; steps 5-6 only
+ * write. In the two-wave form one wave runs 1-4 on window i + 1 while the
+ * other runs 5-6 on window i. A sequence the window cannot hold (a length
+ * with two or more extension bytes, a Snappy literal over 256 bytes) is
+ * decided by the exact scalar ladder and copied wave-wide, global to global.
+ */
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -46,8 +47,6 @@
 namespace cudec_amd {
 namespace {
 
-constexpr int kWavesPerBlock = 4;
-constexpr int kBlock = kWavesPerBlock * kWave;
 constexpr uint32_t kWin = 512;            /* candidate positions per window */
 constexpr uint32_t kCand = kWin / kWave;  /* candidates per lane */
 constexpr uint32_t kStage = 1024;         /* staged bytes used by a window */
@@ -484,10 +483,8 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
     const uint64_t lit_at = D0 + drel;
     const uint64_t m_at = lit_at + lit_len;
     const bool wide_lit = lit_len > kLaneLit;
-    /* phase 1: literals, 16 bytes a step while whole steps fit, then 4
-     * bytes a step; the overrun of at most 3 bytes lands in this member's
-     * own match (>= 4) or, for the terminal literal, past the group where
-     * no history read looks */
+    /* phase 1: literals, 16 bytes a step, the last step exact (a Snappy
+     * literal has no match after it to absorb an overrun) */
     if (active && !wide_lit) {
         uint32_t i0 = 0;
         for (; i0 + 16 <= lit_len; i0 += 16) {
@@ -502,17 +499,10 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
                 }
             }
         }
-        for (uint32_t i = i0; i < lit_len; i += 4) {
-            uint32_t v;
-            __builtin_memcpy(&v, cw + lit + i, 4);
-            const uint32_t r = ring_index(da, lit_at + i);
-            if (r <= kRing - 4) {
-                __builtin_memcpy(&w.ring[r], &v, 4);
-            } else {
-                for (uint32_t b = 0; b < 4; b++) {
-                    w.ring[(r + b) & kRingMask] = (uint8_t)(v >> (8 * b));
-                }
-            }
+        if (i0 < lit_len) {
+            uint4 v;
+            __builtin_memcpy(&v, cw + lit + i0, 16);
+            ring_put(w, ring_index(da, lit_at + i0), v, lit_len - i0);
         }
     }
     wave_order();
@@ -764,6 +754,234 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, const uint8_t* sr
     return out;
 }
 
+/* ------------------------------------------------- Snappy window parse */
+
+/* Longest literal a Snappy window resolves; longer ones take the scalar
+ * ladder. A window literal therefore ends below 512 + 5 + 256. */
+constexpr uint32_t kSnappyWinLit = 256;
+static_assert(kWin + 5 + kSnappyWinLit + 16 <= kStageBuf, "stage covers a Snappy window");
+
+/* The element a tag at window position c would encode
+ * (src/snappy_block.h:288-435): a literal (lit, lit_len; match_len 0) or a
+ * copy-1/2/4 (match_len, offset; lit_len 0). One 8-byte read at the tag. */
+__device__ __forceinline__ Cand parse_candidate_snappy(const uint8_t* cw, uint32_t c) {
+    Cand k;
+    uint64_t h;
+    __builtin_memcpy(&h, cw + c, 8);
+    const uint32_t tag = (uint32_t)h & 0xFF;
+    k.tok = tag;
+    k.complex = false;
+    if ((tag & 3) == 0) {
+        const uint32_t len6 = tag >> 2;
+        uint32_t header = 1;
+        uint64_t length = (uint64_t)len6 + 1;
+        if (len6 > 59) {
+            const uint32_t extra = len6 - 59;
+            header += extra;
+            const uint64_t bytes = h >> 8;
+            length = (extra == 4 ? (bytes & 0xFFFFFFFFull) : (bytes & ((1ull << (8 * extra)) - 1))) + 1;
+        }
+        if (length > kSnappyWinLit) {
+            k.complex = true;
+            length = 0;
+        }
+        k.lit = c + header;
+        k.lit_len = (uint32_t)length;
+        k.match_len = 0;
+        k.offset = 0;
+        k.next = c + header + (uint32_t)length;
+    } else {
+        const uint32_t kind = tag & 3;
+        const uint32_t ob = kind == 1 ? 1 : (kind == 2 ? 2 : 4);
+        uint32_t offset = (uint32_t)(h >> 8);
+        offset &= ob == 1 ? 0xFFu : (ob == 2 ? 0xFFFFu : 0xFFFFFFFFu);
+        uint32_t length;
+        if (kind == 1) {
+            offset |= (tag >> 5) << 8;
+            length = 4 + ((tag >> 2) & 7u);
+        } else {
+            length = (tag >> 2) + 1;
+        }
+        k.lit = c + 1 + ob;
+        k.lit_len = 0;
+        k.match_len = length;
+        k.offset = offset;
+        k.next = c + 1 + ob;
+    }
+    return k;
+}
+
+/* As parse_window, for Snappy elements after the preamble: every member's
+ * rungs run in parallel (every reject is CORRUPT_INPUT, so the first
+ * failing member in stream order decides), and the member whose position
+ * is exactly the end of the source ends the chunk when the output so far
+ * equals the declared length. */
+__device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, const uint8_t* src,
+                                                      uint64_t src_size, uint64_t cap,
+                                                      uint64_t declared, uint64_t& sp,
+                                                      uint64_t& dp, uint32_t lane) {
+    Parsed out;
+    out.kind = kParseFail;
+    out.status = kOk;
+    out.done = false;
+    out.delta = stage_into(stage, src, src_size, sp, lane);
+    const uint8_t* cw = stage + out.delta;
+
+    uint32_t link[kCand];
+#pragma unroll
+    for (uint32_t k = 0; k < kCand; k++) {
+        const Cand c = parse_candidate_snappy(cw, k * kWave + lane);
+        link[k] = c.complex ? kStop : c.next;
+    }
+    uint32_t pos = 0, count = 0;
+    uint32_t member = 0;
+    bool stopped = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kCand; k++) {
+        while (!stopped && pos < (k + 1) * kWave && count < kWave) {
+            const uint32_t nx = __builtin_amdgcn_readlane(link[k], pos - k * kWave);
+            if (nx == kStop) {
+                stopped = true;
+                break;
+            }
+            member = lane == count ? pos : member;
+            count++;
+            pos = nx;
+        }
+    }
+    count = uniform32(count);
+    pos = uniform32(pos);
+
+    bool slow = count == 0;
+    if (!slow) {
+        const bool is_member = lane < count;
+        const Cand c = parse_candidate_snappy(cw, is_member ? member : 0);
+        const uint64_t s = sp + member;
+        const bool at_end = is_member && s >= src_size;
+        const uint32_t out_len = is_member && !at_end ? c.lit_len + c.match_len : 0;
+        const uint32_t incl = wave_inclusive_sum(out_len, lane);
+        const uint32_t drel = incl - out_len;
+        const uint64_t d0 = dp + drel;
+        int ev = kOk;
+        if (is_member) {
+            if (at_end) {
+                ev = s == src_size && d0 == declared ? kOk : kCorrupt;
+            } else if (d0 > declared) {
+                ev = kCorrupt;
+            } else if (c.match_len == 0) {
+                const uint64_t header = c.lit - member;
+                if (header > src_size - s) {
+                    ev = kCorrupt;
+                } else if (c.lit_len > src_size - s - header) {
+                    ev = kCorrupt;
+                } else if (c.lit_len > declared - d0) {
+                    ev = kCorrupt;
+                }
+            } else {
+                const uint64_t header = c.lit - member;
+                if (header > src_size - s) {
+                    ev = kCorrupt;
+                } else if (c.offset == 0 || (uint64_t)c.offset > d0) {
+                    ev = kCorrupt;
+                } else if (c.match_len > declared - d0) {
+                    ev = kCorrupt;
+                }
+            }
+        }
+        const bool cut = is_member && !at_end && drel + out_len > kBatchMax;
+        const bool flag = is_member && (cut || ev != kOk || at_end);
+        const uint64_t flags = __ballot(flag);
+        uint32_t take = count;
+        bool ended = false;
+        int status = kOk;
+        if (flags != 0) {
+            const uint32_t j = __builtin_ctzll(flags);
+            take = j;
+            const bool j_cut = __builtin_amdgcn_readlane((uint32_t)cut, j) != 0;
+            const int j_ev = (int)__builtin_amdgcn_readlane((uint32_t)ev, j);
+            if (j_ev != kOk) {
+                status = j_ev;
+            } else if (j_cut) {
+                slow = j == 0;
+            } else {
+                ended = true;
+            }
+        }
+        take = uniform32(take);
+        ended = uniform32(ended) != 0;
+        slow = uniform32(slow) != 0;
+        status = (int)uniform32((uint32_t)status);
+        if (status != kOk) {
+            out.status = status;
+            return out;
+        }
+        if (!slow && (take != 0 || ended)) {
+            out.kind = kParseGroup;
+            out.g.lit = c.lit;
+            out.g.lit_len = c.lit_len;
+            out.g.match_len = c.match_len;
+            out.g.offset = c.offset;
+            out.g.drel = drel;
+            out.g.take = take;
+            out.g.terminal = false;
+            out.g.end_rel = take == 0 ? 0u : uniform32(__builtin_amdgcn_readlane(incl, take - 1));
+            const uint32_t consumed = take == count ? pos : __builtin_amdgcn_readlane(member, take);
+            sp = uniform64(ended ? src_size : sp + consumed);
+            dp = uniform64(dp + out.g.end_rel);
+            out.done = ended;
+            return out;
+        }
+    }
+    /* one element by the exact scalar ladder */
+    SnappyCursor cur{src, src_size, cap, sp, dp, declared};
+    SnappyStep e;
+    bool last = false;
+    const int st = (int)uniform32((uint32_t)snappy_next(cur, e, last));
+    if (st != kOk) {
+        out.status = st;
+        return out;
+    }
+    out.kind = kParseSlow;
+    out.done = uniform32(last) != 0;
+    out.q.lit_src = e.is_copy ? 0 : e.from;
+    out.q.lit_dst = e.to;
+    out.q.lit_len = e.is_copy ? 0 : e.length;
+    out.q.match_dst = e.to + (e.is_copy ? 0 : e.length);
+    out.q.match_len = e.is_copy ? e.length : 0;
+    out.q.offset = e.is_copy ? (uint32_t)e.offset : 1u;
+    sp = uniform64(cur.sp);
+    dp = uniform64(cur.dp);
+    return out;
+}
+
+/* A window of either format. */
+template <int kFormat>
+__device__ __forceinline__ Parsed parse_any(uint8_t* stage, const uint8_t* src, uint64_t src_size,
+                                            uint64_t cap, uint64_t declared, uint64_t& sp,
+                                            uint64_t& dp, uint32_t lane) {
+    if (kFormat == kFormatSnappy) {
+        return parse_window_snappy(stage, src, src_size, cap, declared, sp, dp, lane);
+    }
+    return parse_window(stage, src, src_size, cap, sp, dp, lane);
+}
+
+/* Start of a chunk: the Snappy preamble and its declared length against the
+ * capacity (src/snappy_block.h:190-263); nothing for LZ4. */
+template <int kFormat>
+__device__ __forceinline__ int chunk_begin(const uint8_t* src, uint64_t src_size, uint64_t cap,
+                                           uint64_t& sp, uint64_t& declared) {
+    sp = 0;
+    declared = 0;
+    if (kFormat != kFormatSnappy) {
+        return kOk;
+    }
+    SnappyCursor c{src, src_size, cap, 0, 0, 0};
+    const int st = (int)uniform32((uint32_t)snappy_begin(c));
+    sp = uniform64(c.sp);
+    declared = uniform64(c.declared);
+    return st;
+}
+
 /* Executes a sequence the window could not hold: everything in the ring
  * goes out first, then the literal and the match run global to global. */
 __device__ __forceinline__ void execute_slow(WaveLds& w, const uint8_t* src, uint8_t* dst,
@@ -789,7 +1007,7 @@ struct alignas(16) PipeRec {
     uint16_t lit[kWave];
     uint16_t lit_len[kWave];
     uint16_t match_len[kWave];
-    uint16_t offset[kWave];
+    uint32_t offset[kWave];
     uint16_t drel[kWave];
     uint32_t kind;
     int32_t status;
@@ -823,18 +1041,19 @@ struct ChunkOut {
 };
 
 /* One wave per chunk: parse a window, then execute it, in turn. */
-__device__ __forceinline__ ChunkOut decode_lz4_chunk(WaveLds& w, const uint8_t* src,
-                                                     uint64_t src_size, uint8_t* dst,
-                                                     uint64_t cap, uint32_t lane) {
-    uint64_t sp = 0, pdp = 0, dp = 0, flushed = 0, ring_lo = 0;
+template <int kFormat>
+__device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, const uint8_t* src,
+                                                 uint64_t src_size, uint8_t* dst, uint64_t cap,
+                                                 uint32_t lane) {
+    uint64_t sp = 0, declared = 0, pdp = 0, dp = 0, flushed = 0, ring_lo = 0;
     uint64_t fuel = src_size == UINT64_MAX ? UINT64_MAX : src_size + 2;
-    int status = kOk;
-    for (;;) {
+    int status = chunk_begin<kFormat>(src, src_size, cap, sp, declared);
+    while (status == kOk) {
         if (fuel-- == 0) {
             status = kCorrupt;
             break;
         }
-        const Parsed pr = parse_window(w.stage, src, src_size, cap, sp, pdp, lane);
+        const Parsed pr = parse_any<kFormat>(w.stage, src, src_size, cap, declared, sp, pdp, lane);
         if (pr.kind == kParseFail) {
             status = pr.status;
             break;
@@ -912,71 +1131,10 @@ __global__ void queue_fill(const size_t* sizes, const size_t* caps, size_t n, Qu
     }
 }
 
-template <int kFormat>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
-    tuned_batch(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
-                const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
-                const uint32_t* order) {
-    __shared__ WaveLds lds[kWavesPerBlock];
-    const uint32_t lane = __lane_id();
-    WaveLds& w = lds[threadIdx.x / kWave];
-    /* The loop skeleton carries no lane-divergent branch: every lane takes
-     * part in the queue atomic (only lane 0 adds 1; the others add 0 and see
-     * the same counter), the index is broadcast from the first lane, and
-     * every lane stores the same result record. A lane-0-only branch at the
-     * loop's top and bottom lets the compiler fold both into a nested
-     * exec-masked loop whose exit holds only lane 0. */
-    for (;;) {
-        const uint32_t mine = atomicAdd(&q->next, lane == 0 ? 1u : 0u);
-        const uint32_t idx = uniform32(mine);
-        if (idx >= n) {
-            break;
-        }
-        const uint64_t chunk = order[idx];
-        const uint8_t* src = uniform_ptr(static_cast<const uint8_t*>(srcs[chunk]));
-        uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[chunk]));
-        const uint64_t src_size = uniform64(src_sizes[chunk]);
-        const uint64_t cap = uniform64(caps[chunk]);
-        int status = kOk;
-        uint64_t produced = 0;
-        if (kFormat == kFormatLz4) {
-            const ChunkOut o = decode_lz4_chunk(w, src, src_size, dst, cap, lane);
-            status = o.status;
-            produced = o.produced;
-        } else {
-            SnappyCursor c{src, src_size, cap, 0, 0, 0};
-            SnappyStep e;
-            bool done = false;
-            uint64_t fuel = src_size == UINT64_MAX ? UINT64_MAX : src_size + 1;
-            status = snappy_begin(c);
-            while (status == kOk) {
-                status = snappy_next(c, e, done);
-                if (status != kOk) {
-                    break;
-                }
-                if (e.is_copy) {
-                    wave_match_global(w, dst, e.to, e.offset, e.length, cap, lane);
-                } else {
-                    wave_copy_global(dst + e.to, src + e.from, e.length, lane);
-                }
-                if (done || fuel-- == 0) {
-                    break;
-                }
-            }
-            if (status == kOk && !done) {
-                status = kCorrupt;
-            }
-            produced = c.dp;
-        }
-        status = (int)uniform32((uint32_t)status);
-        write_result(results, chunk, status, uniform64(produced));
-    }
-}
-
 constexpr int kPipeBlock = 2 * kWave;
 constexpr uint32_t kNoFin = 0xFFFFFFFFu;
 
-/* LZ4, two waves per chunk. Wave 0 parses window i into record i % 2 (and
+/* Two waves per chunk. Wave 0 parses window i into record i % 2 (and
  * stage buffer i % 2) while wave 1 executes record i - 1; one barrier
  * separates the steps. Parsing reads only the source and decides every
  * status; execution only writes the ring and the destination, so the
@@ -985,20 +1143,26 @@ constexpr uint32_t kNoFin = 0xFFFFFFFFu;
  * both waves read the chunk index and the final step from LDS after a
  * barrier, and a barrier opens every chunk so that no control word is
  * rewritten while the other wave may still read it. */
+template <int kFormat>
 __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu(4)))
-    piped_lz4(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
-              const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
-              const uint32_t* order, uint32_t pipe_bin) {
+    piped_batch(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
+                const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
+                const uint32_t* order, int32_t pipe_bin) {
     __shared__ PipeLds L;
     const uint32_t lane = __lane_id();
     const bool parser = uniform32(threadIdx.x / kWave) == 0;
     uint32_t split_idx = 0xFFFFFFFFu; /* first chunk decoded one wave per chunk */
+    /* pipe_bin: chunks whose cost bin is at most this run on two waves; -1:
+     * none (the first chunk splits the block) */
     for (;;) {
         /* both waves are done reading the previous chunk's control words
          * (chunk_idx, fin) before the parser writes the next chunk's */
         __syncthreads();
         if (parser) {
-            /* every lane takes part; only lane 0 adds (see tuned_batch) */
+            /* every lane takes part in the queue atomic and only lane 0 adds:
+             * a lane-0-only branch at a persistent loop's top and bottom lets
+             * the compiler fold both into an exec-masked inner loop that only
+             * lane 0 can leave */
             const uint32_t mine = atomicAdd(&q->next, lane == 0 ? 1u : 0u);
             L.chunk_idx = uniform32(mine);
             L.fin = kNoFin;
@@ -1013,30 +1177,33 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
         uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[chunk]));
         const uint64_t src_size = uniform64(src_sizes[chunk]);
         const uint64_t cap = uniform64(caps[chunk]);
-        if (cost_bin(src_size, cap) > pipe_bin) {
+        if ((int32_t)cost_bin(src_size, cap) > pipe_bin) {
             /* the queue runs largest cost first: every chunk from here on
              * is cheaper too, so the block splits for good */
             split_idx = idx;
             break;
         }
-        uint64_t sp = 0, dp = 0, flushed = 0, ring_lo = 0;
+        uint64_t sp = 0, declared = 0, dp = 0, flushed = 0, ring_lo = 0;
         uint64_t fuel = src_size == UINT64_MAX ? UINT64_MAX : src_size + 2;
         int status = kOk;
         bool pdone = false;
+        /* the parser's preamble verdict (Snappy), handed over as a failed
+         * window record when it is not OK */
+        const int begun = parser ? chunk_begin<kFormat>(src, src_size, cap, sp, declared) : kOk;
         for (uint32_t step = 0;; step++) {
             const uint32_t b = step & 1;
             if (parser) {
                 if (!pdone) {
                     PipeRec& r = L.rec[b];
                     Parsed pr;
-                    if (fuel-- == 0) {
+                    if (begun != kOk || fuel-- == 0) {
                         pr.kind = kParseFail;
-                        pr.status = kCorrupt;
+                        pr.status = begun != kOk ? begun : kCorrupt;
                         pr.done = false;
                         pr.delta = 0;
                     } else {
-                        pr = parse_window(b ? L.stage1 : L.w.stage, src, src_size, cap, sp, dp,
-                                          lane);
+                        pr = parse_any<kFormat>(b ? L.stage1 : L.w.stage, src, src_size, cap,
+                                                declared, sp, dp, lane);
                     }
                     r.kind = pr.kind;
                     r.status = pr.status;
@@ -1045,7 +1212,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         r.lit[lane] = (uint16_t)pr.g.lit;
                         r.lit_len[lane] = (uint16_t)pr.g.lit_len;
                         r.match_len[lane] = (uint16_t)pr.g.match_len;
-                        r.offset[lane] = (uint16_t)pr.g.offset;
+                        r.offset[lane] = pr.g.offset;
                         r.drel[lane] = (uint16_t)pr.g.drel;
                         r.delta = pr.delta;
                         r.take = pr.g.take;
@@ -1118,7 +1285,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
         uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[chunk]));
         const uint64_t src_size = uniform64(src_sizes[chunk]);
         const uint64_t cap = uniform64(caps[chunk]);
-        const ChunkOut o = decode_lz4_chunk(w, src, src_size, dst, cap, lane);
+        const ChunkOut o = decode_chunk<kFormat>(w, src, src_size, dst, cap, lane);
         write_result(results, chunk, (int)uniform32((uint32_t)o.status), uniform64(o.produced));
         idx = uniform32(atomicAdd(&q->next, lane == 0 ? 1u : 0u));
     }
@@ -1129,31 +1296,29 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
  * stream of launches does not return memory to the driver between them. */
 struct DeviceState {
     int cus = 0;
-    int blocks_per_cu = 0;       /* tuned_batch (one wave per chunk) */
-    int pipe_blocks_per_cu = 0;  /* piped_lz4 (two waves per chunk) */
+    int pipe_blocks_per_cu = 0;  /* piped_batch (two-wave blocks) */
     uint64_t lz4_pipe_min_cost = kDefaultPipeMinCost; /* two waves at or above this cost */
     hipMemPool_t pool = nullptr;
     bool ready = false;
 };
 constexpr int kMaxDevices = 64;
 
-/* The cost bin at or below which (costlier chunks) the pipelined form runs a
- * chunk on two waves; UINT32_MAX-free mapping of a byte threshold. */
-uint32_t pipe_bin_for(uint64_t min_cost) {
+/* The cost bin at or below which (costlier chunks) a chunk runs on two
+ * waves: the bin of a chunk whose cost is exactly min_cost (host mirror of
+ * cost_bin). 0 admits every bin; UINT64_MAX none (-1). */
+int32_t pipe_bin_for(uint64_t min_cost) {
     if (min_cost == 0) {
-        return kBins - 1;   /* every chunk */
+        return kBins - 1;
     }
     if (min_cost == UINT64_MAX) {
-        return 0xFFFFFFFFu; /* none: the first chunk splits the block */
+        return -1;
     }
-    /* the bin of a chunk whose cost is exactly min_cost (host mirror of
-     * cost_bin); costlier chunks have bins at or below it */
     const uint64_t c = min_cost;
     if (c < 4) {
-        return (uint32_t)(kBins - 1 - c);
+        return (int32_t)(kBins - 1 - c);
     }
     const uint32_t lg = 63u - (uint32_t)__builtin_clzll(c);
-    return (uint32_t)(kBins - 1 - (lg * 4 + (uint32_t)((c >> (lg - 2)) & 3)));
+    return (int32_t)(kBins - 1 - (lg * 4 + (uint32_t)((c >> (lg - 2)) & 3)));
 }
 DeviceState g_devices[kMaxDevices];
 std::mutex g_devices_lock;
@@ -1175,12 +1340,8 @@ hipError_t device_state(DeviceState** out) {
             return e;
         }
         int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, tuned_batch<kFormatLz4>, kBlock, 0);
-        if (e != hipSuccess) {
-            return e;
-        }
-        d.blocks_per_cu = per > 0 ? per : 1;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_lz4, kPipeBlock, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4>, kPipeBlock,
+                                                         0);
         if (e != hipSuccess) {
             return e;
         }
@@ -1262,28 +1423,19 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     hipLaunchKernelGGL(queue_count, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q);
     hipLaunchKernelGGL(queue_scan, dim3(1), dim3(1), 0, stream, q);
     hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q, order);
-    const uint32_t pipe_bin = pipe_bin_for(min_cost);
-    if (format == kFormatLz4 && pipe_bin != 0xFFFFFFFFu) {
-        /* one chunk per two-wave block at a time */
-        size_t grid = (size_t)dev->cus * (size_t)dev->pipe_blocks_per_cu;
-        if (grid > n) {
-            grid = n;
-        }
-        hipLaunchKernelGGL(piped_lz4, dim3((unsigned)grid), dim3(kPipeBlock), 0, stream, srcs,
-                           src_sizes, dsts, caps, n, results, q, order, pipe_bin);
+    const int32_t pipe_bin = pipe_bin_for(min_cost);
+    /* two-wave blocks; a block splits into two one-wave decoders when it
+     * meets a chunk below the threshold */
+    size_t grid = (size_t)dev->cus * (size_t)dev->pipe_blocks_per_cu;
+    if (grid > n) {
+        grid = n;
+    }
+    if (format == kFormatSnappy) {
+        hipLaunchKernelGGL(piped_batch<kFormatSnappy>, dim3((unsigned)grid), dim3(kPipeBlock), 0,
+                           stream, srcs, src_sizes, dsts, caps, n, results, q, order, pipe_bin);
     } else {
-        size_t grid = (size_t)dev->cus * (size_t)dev->blocks_per_cu;
-        const size_t need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
-        if (grid > need) {
-            grid = need;
-        }
-        if (format == kFormatSnappy) {
-            hipLaunchKernelGGL(tuned_batch<kFormatSnappy>, dim3((unsigned)grid), dim3(kBlock), 0,
-                               stream, srcs, src_sizes, dsts, caps, n, results, q, order);
-        } else {
-            hipLaunchKernelGGL(tuned_batch<kFormatLz4>, dim3((unsigned)grid), dim3(kBlock), 0,
-                               stream, srcs, src_sizes, dsts, caps, n, results, q, order);
-        }
+        hipLaunchKernelGGL(piped_batch<kFormatLz4>, dim3((unsigned)grid), dim3(kPipeBlock), 0,
+                           stream, srcs, src_sizes, dsts, caps, n, results, q, order, pipe_bin);
     }
     e = hipGetLastError();
     const hipError_t f = hipFreeAsync(scratch, stream);
