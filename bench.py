@@ -50,6 +50,9 @@ def parse_args():
     p.add_argument("--cpu-threads", type=int, default=None)
     p.add_argument("--traffic-file", default=None,
                    help="JSON with measured HBM bytes per launch (rocprofv3 PMC)")
+    p.add_argument("--malformed-pct", type=float, default=None,
+                   help="percent of chunks made malformed by a seeded truncation or a "
+                        "single-byte flip after the preamble (default: 2 for snappy, 0 for lz4)")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="before timing, run a plain elementwise copy (torch.neg over int32) "
                         "of this many MiB three times: the known-size kernel that HBM "
@@ -82,24 +85,65 @@ def build_batch(args):
     log(f"generated + compressed {n} chunks in {time.perf_counter() - t0:.1f}s: "
         f"{b.total_uncompressed / 2**30:.2f} GiB -> {b.total_compressed / 2**30:.2f} GiB "
         f"({workload.codec_version(codec)})")
-    return b, name
+    pct = args.malformed_pct if args.malformed_pct is not None else \
+        (2.0 if args.format == "snappy" else 0.0)
+    malformed = malform(b, pct, args.seed)
+    if len(malformed):
+        name += f", {pct:g}% malformed ({len(malformed)} chunks: truncation or byte flip)"
+    return b, name, malformed
 
 
-def time_form(db, fmt, form, steps, warmup, stream, orig_dev, sizes_np):
+def malform(b, pct, seed):
+    """Makes pct percent of the chunks malformed in place, seeded: half lose
+    1-16 trailing bytes, half get one byte after the first 5 XOR-ed with a
+    nonzero value. Returns the sorted indices. A flip may leave a chunk
+    valid; the decoders are then held to the CPU reference's verdict."""
+    import numpy as np
+    k = int(round(b.n * pct / 100.0))
+    if k == 0:
+        return np.zeros(0, dtype=np.int64)
+    rng = np.random.default_rng(seed ^ 0xBAD)
+    idx = np.sort(rng.choice(b.n, size=k, replace=False))
+    for i in idx:
+        size = int(b.comp_size[i])
+        if size < 2:
+            continue
+        if rng.integers(0, 2) == 0:
+            b.comp_size[i] = size - int(rng.integers(1, min(16, size - 1) + 1))
+        else:
+            at = int(b.comp_off[i]) + int(rng.integers(min(5, size - 1), size))
+            b.comp[at] ^= np.uint8(rng.integers(1, 256))
+    return idx
+
+
+def time_form(db, fmt, form, steps, warmup, stream, orig_dev, batch, valid):
+    """Warmup, a byte-exact check of every intact chunk (status OK,
+    bytes_written = size, output = original), then `steps` event-timed
+    launches. Returns (wall seconds, per-launch ms, the checked results)."""
     import numpy as np
     import torch
     for _ in range(warmup):
         st = db.run(fmt, form, stream=stream)
         assert st == 0, f"{form} submission failed: {st}"
     torch.cuda.synchronize()
-    # verify every chunk of this form, byte for byte
-    res = db.results()
-    ok = bool(np.all(res["status"] == 0)) and bool(np.array_equal(res["bytes_written"], sizes_np)) \
-        and bool(np.all(res["reserved"] == 0))
-    same = bool(torch.equal(db.d_out[: orig_dev.numel()], orig_dev))
+    res = db.results().copy()
+    ok = bool(np.all(res["status"][valid] == 0)) and \
+        bool(np.array_equal(res["bytes_written"][valid], batch.sizes[valid])) and \
+        bool(np.all(res["reserved"] == 0))
+    if bool(np.all(valid)):
+        same = bool(torch.equal(db.d_out[: orig_dev.numel()], orig_dev))
+    else:
+        # malformed chunks may leave anything in their slots: take those
+        # slots from the decoder's output and compare everything else
+        expect = orig_dev.clone()
+        for i in np.nonzero(~valid)[0]:
+            o, n = int(batch.orig_off[i]), int(batch.sizes[i])
+            expect[o:o + n] = db.d_out[o:o + n]
+        same = bool(torch.equal(db.d_out[: orig_dev.numel()], expect))
+        del expect
     if not (ok and same):
-        bad = int(np.sum(res["status"] != 0))
-        raise SystemExit(f"[bench] {form}: verification FAILED (non-OK chunks: {bad}, "
+        bad = int(np.sum(res["status"][valid] != 0))
+        raise SystemExit(f"[bench] {form}: verification FAILED (intact non-OK chunks: {bad}, "
                          f"arena equal: {same})")
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
@@ -113,13 +157,15 @@ def time_form(db, fmt, form, steps, warmup, stream, orig_dev, sizes_np):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    return wall, kernel_ms
+    return wall, kernel_ms, res
 
 
-def cpu_baseline(batch, fmt, threads):
+def cpu_baseline(batch, fmt, threads, valid):
     """The reference ladder compiled from its own source (oracle/_ref) when
     that build is present, else the C oracle: `threads` host threads, one
-    pass over the whole batch into pre-sized buffers, decode calls only."""
+    pass over the whole batch into pre-sized buffers, decode calls only.
+    Returns the baseline record and every chunk's (status, bytes_written),
+    which the GPU results are held to."""
     import numpy as np
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libcudec_ref.so")
     port_path = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -136,6 +182,8 @@ def cpu_baseline(batch, fmt, threads):
     fn.argtypes = [ctypes.c_void_p, u64, ctypes.c_void_p, u64, ctypes.POINTER(u64)] + extra
     fn.restype = ctypes.c_int32
     out = np.empty(batch.total_uncompressed, dtype=np.uint8)
+    status = np.zeros(batch.n, dtype=np.int32)
+    written = np.zeros(batch.n, dtype=np.uint64)
     cbase = batch.comp.ctypes.data
     obase = out.ctypes.data
     n = batch.n
@@ -143,27 +191,37 @@ def cpu_baseline(batch, fmt, threads):
 
     def work(t):
         w = u64(0)
-        bad = 0
         for i in range(bounds[t], bounds[t + 1]):
             args = [cbase + int(batch.comp_off[i]), int(batch.comp_size[i]),
                     obase + int(batch.orig_off[i]), int(batch.sizes[i]), ctypes.byref(w)]
             if extra:
                 args.append(None)
-            if fn(*args) != 0 or w.value != int(batch.sizes[i]):
-                bad += 1
-        return bad
+            status[i] = fn(*args)
+            written[i] = w.value if status[i] == 0 else 0
+        return 0
 
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
         t0 = time.perf_counter()
-        bad = sum(ex.map(work, range(threads)))
+        list(ex.map(work, range(threads)))
         secs = time.perf_counter() - t0
-    if bad or not np.array_equal(out, batch.orig):
+    intact_ok = bool(np.all(status[valid] == 0)) and \
+        bool(np.array_equal(written[valid], batch.sizes[valid]))
+    if intact_ok and bool(np.all(valid)):
+        intact_ok = bool(np.array_equal(out, batch.orig))
+    elif intact_ok:
+        for i in np.nonzero(valid)[0]:
+            o, m = int(batch.orig_off[i]), int(batch.sizes[i])
+            if not np.array_equal(out[o:o + m], batch.orig[o:o + m]):
+                intact_ok = False
+                break
+    if not intact_ok:
         raise SystemExit("[bench] CPU baseline produced wrong output")
-    return {"value": round(batch.total_uncompressed / secs / 1e9, 3), "unit": "GB/s (uncompressed)",
-            "cores": threads, "kind": kind,
-            "sample": f"one pass over the full batch ({n} chunks, "
-                      f"{batch.total_uncompressed / 2**30:.2f} GiB out), {threads} threads, "
-                      f"{secs:.2f} s wall"}
+    rec = {"value": round(batch.total_uncompressed / secs / 1e9, 3), "unit": "GB/s (uncompressed)",
+           "cores": threads, "kind": kind,
+           "sample": f"one pass over the full batch ({n} chunks, "
+                     f"{batch.total_uncompressed / 2**30:.2f} GiB out), {threads} threads, "
+                     f"{secs:.2f} s wall"}
+    return rec, status, written
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "lz4_uneven_hbm_traffic.md")
@@ -255,7 +313,9 @@ def main():
     from cudec_amd.device import DeviceBatch
 
     torch.cuda.init()
-    batch, workload_name = build_batch(args)
+    batch, workload_name, malformed = build_batch(args)
+    valid = np.ones(batch.n, dtype=bool)
+    valid[malformed] = False
     t0 = time.perf_counter()
     db = DeviceBatch(batch.comp, batch.comp_off, batch.comp_size, batch.sizes,
                      out_off=batch.orig_off)
@@ -295,10 +355,11 @@ def main():
                                round(ms, 3) for c, ms in sweep}}
         log(f"autotune ({time.perf_counter() - t0:.1f}s): best threshold {best}; {tuning['sweep_ms']}")
     for form in forms:
-        wall, kms = time_form(db, args.format, form, args.steps, args.warmup, stream, orig_dev,
-                              batch.sizes)
+        wall, kms, res = time_form(db, args.format, form, args.steps, args.warmup, stream,
+                                   orig_dev, batch, valid)
         avg_ms = sum(kms) / len(kms)
         measured[form] = {"wall_s": wall, "kernel_ms": kms, "avg_kernel_ms": avg_ms,
+                          "results": res,
                           "gbps": uncomp * args.steps / wall / 1e9,
                           "kernel_gbps": uncomp / (avg_ms / 1e3) / 1e9,
                           "algo_gbps": algo / (avg_ms / 1e3) / 1e9}
@@ -307,10 +368,19 @@ def main():
             f"= {100 * measured[form]['algo_gbps'] / HBM_PEAK_GBPS:.1f}% of HBM peak")
     head = measured[forms[0]]
     cpu = None
+    malformed_checked = False
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(batch, args.format, threads)
+        cpu, cpu_status, cpu_written = cpu_baseline(batch, args.format, threads, valid)
         log(f"cpu baseline: {cpu}")
+        # every chunk, malformed ones included: the GPU's status and
+        # bytes_written equal the CPU reference's
+        for form in forms:
+            r = measured[form]["results"]
+            if not (np.array_equal(r["status"], cpu_status) and
+                    np.array_equal(r["bytes_written"], cpu_written)):
+                raise SystemExit(f"[bench] {form}: results differ from the CPU reference")
+        malformed_checked = True
     serving = serving_leg(args) if args.serve_rates else None
     traffic = None
     if args.traffic_file and os.path.exists(args.traffic_file):
@@ -351,6 +421,12 @@ def main():
     line["avg_kernel_ms"] = round(head["avg_kernel_ms"], 4)
     if tuning is not None:
         line["autotune"] = tuning
+    if len(malformed):
+        r = head["results"]
+        line["malformed"] = {"chunks": int(len(malformed)),
+                             "status_counts": {str(k): int(v) for k, v in zip(
+                                 *np.unique(r["status"][malformed], return_counts=True))},
+                             "checked_against_cpu_reference": malformed_checked}
     if serving is not None:
         line["serving"] = serving
         line["p99_request_latency_ms"] = serving["rates"][-1]["p99_ms"]
