@@ -569,7 +569,9 @@ struct Parsed {
     bool done;        /* the chunk ends after this group / sequence */
     uint32_t delta;
     Group g;
-    Lz4Step q;
+    Lz4Step q;        /* kSlow: the sequence; kGroup with lead: runs first */
+    bool lead;        /* kGroup: q precedes the group */
+    uint64_t lead_dp; /* output position after q */
 };
 
 /* Stages the window at sp into `stage`, speculates, walks and resolves it
@@ -582,6 +584,9 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, const uint8_t* sr
     out.kind = kParseFail;
     out.status = kOk;
     out.done = false;
+    out.lead = false;
+    out.lead_dp = 0;
+    const uint64_t base = sp; /* window start; member positions are relative to it */
     out.delta = stage_into(stage, src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
 
@@ -593,8 +598,34 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, const uint8_t* sr
         link[k] = c.complex ? kStop : c.next;
     }
 
+    /* a window that opens on a sequence it cannot hold: the exact scalar
+     * ladder decides that one, and the chain goes on from just past it
+     * (the speculated links stay valid) */
+    uint32_t start = 0;
+    if (uniform32(__builtin_amdgcn_readlane(link[0], 0)) == kStop) {
+        Lz4Cursor cur{src, src_size, cap, sp, dp};
+        bool last = false;
+        const int st = (int)uniform32((uint32_t)lz4_next(cur, out.q, last));
+        if (st != kOk) {
+            out.status = st;
+            return out;
+        }
+        sp = uniform64(cur.sp);
+        dp = uniform64(cur.dp);
+        out.lead = true;
+        out.lead_dp = dp;
+        const uint64_t adv = sp - base;
+        if (uniform32(last) != 0 || adv >= kWin) {
+            out.kind = kParseSlow;
+            out.lead = false;
+            out.done = uniform32(last) != 0;
+            return out;
+        }
+        start = (uint32_t)adv;
+    }
+
     /* walk the true chain */
-    uint32_t pos = 0, count = 0;
+    uint32_t pos = start, count = 0;
     uint32_t member = 0; /* lane j: position of member j */
     bool stopped = false;
 #pragma unroll
@@ -625,7 +656,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, const uint8_t* sr
         int ev = kOk;
         bool terminal = false;
         if (is_member) {
-            uint64_t s = sp + member;  /* absolute source position */
+            uint64_t s = base + member;  /* absolute source position */
             if (s >= src_size) {
                 ev = kCorrupt;
             } else {
@@ -729,15 +760,20 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, const uint8_t* sr
             out.g.end_rel = uniform32(__builtin_amdgcn_readlane(incl, last) -
                                       (take_terminal ? __builtin_amdgcn_readlane(c.match_len, last)
                                                      : 0));
-            const uint32_t consumed =
-                take_terminal ? (uint32_t)(src_size - sp)
-                : take == count ? pos
-                                : __builtin_amdgcn_readlane(member, take);
-            sp = uniform64(sp + consumed);
+            const uint64_t next_sp =
+                take_terminal ? src_size
+                              : base + (take == count ? pos : __builtin_amdgcn_readlane(member, take));
+            sp = uniform64(next_sp);
             dp = uniform64(dp + out.g.end_rel);
             out.done = take_terminal;
             return out;
         }
+    }
+    if (out.lead) {
+        /* nothing after the leading sequence fits this window */
+        out.kind = kParseSlow;
+        out.lead = false;
+        return out;
     }
     /* one sequence by the exact scalar ladder */
     Lz4Cursor cur{src, src_size, cap, sp, dp};
@@ -824,6 +860,9 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, const uint
     out.kind = kParseFail;
     out.status = kOk;
     out.done = false;
+    out.lead = false;
+    out.lead_dp = 0;
+    const uint64_t base = sp;
     out.delta = stage_into(stage, src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
 
@@ -833,7 +872,38 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, const uint
         const Cand c = parse_candidate_snappy(cw, k * kWave + lane);
         link[k] = c.complex ? kStop : c.next;
     }
-    uint32_t pos = 0, count = 0;
+    /* a window that opens on a literal it cannot hold: the scalar ladder
+     * decides it and the chain goes on from just past it */
+    uint32_t start = 0;
+    if (uniform32(__builtin_amdgcn_readlane(link[0], 0)) == kStop) {
+        SnappyCursor cur{src, src_size, cap, sp, dp, declared};
+        SnappyStep e;
+        bool last = false;
+        const int st = (int)uniform32((uint32_t)snappy_next(cur, e, last));
+        if (st != kOk) {
+            out.status = st;
+            return out;
+        }
+        out.q.lit_src = e.is_copy ? 0 : e.from;
+        out.q.lit_dst = e.to;
+        out.q.lit_len = e.is_copy ? 0 : e.length;
+        out.q.match_dst = e.to + (e.is_copy ? 0 : e.length);
+        out.q.match_len = e.is_copy ? e.length : 0;
+        out.q.offset = e.is_copy ? (uint32_t)e.offset : 1u;
+        sp = uniform64(cur.sp);
+        dp = uniform64(cur.dp);
+        out.lead = true;
+        out.lead_dp = dp;
+        const uint64_t adv = sp - base;
+        if (uniform32(last) != 0 || adv >= kWin) {
+            out.kind = kParseSlow;
+            out.lead = false;
+            out.done = uniform32(last) != 0;
+            return out;
+        }
+        start = (uint32_t)adv;
+    }
+    uint32_t pos = start, count = 0;
     uint32_t member = 0;
     bool stopped = false;
 #pragma unroll
@@ -856,7 +926,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, const uint
     if (!slow) {
         const bool is_member = lane < count;
         const Cand c = parse_candidate_snappy(cw, is_member ? member : 0);
-        const uint64_t s = sp + member;
+        const uint64_t s = base + member;
         const bool at_end = is_member && s >= src_size;
         const uint32_t out_len = is_member && !at_end ? c.lit_len + c.match_len : 0;
         const uint32_t incl = wave_inclusive_sum(out_len, lane);
@@ -926,11 +996,16 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, const uint
             out.g.terminal = false;
             out.g.end_rel = take == 0 ? 0u : uniform32(__builtin_amdgcn_readlane(incl, take - 1));
             const uint32_t consumed = take == count ? pos : __builtin_amdgcn_readlane(member, take);
-            sp = uniform64(ended ? src_size : sp + consumed);
+            sp = uniform64(ended ? src_size : base + consumed);
             dp = uniform64(dp + out.g.end_rel);
             out.done = ended;
             return out;
         }
+    }
+    if (out.lead) {
+        out.kind = kParseSlow;
+        out.lead = false;
+        return out;
     }
     /* one element by the exact scalar ladder */
     SnappyCursor cur{src, src_size, cap, sp, dp, declared};
@@ -1016,7 +1091,7 @@ struct alignas(16) PipeRec {
     uint32_t take;
     uint32_t terminal;
     uint32_t end_rel;
-    uint32_t pad;
+    uint32_t lead;              /* kGroup: the slow sequence below runs first */
     uint64_t lit_src, lit_dst, lit_len_s, match_dst, offset_s, match_len_s, dp_after;
 };
 
@@ -1059,6 +1134,9 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, const uint8_t* src,
             break;
         }
         if (pr.kind == kParseGroup) {
+            if (pr.lead) {
+                execute_slow(w, src, dst, cap, pr.q, pr.lead_dp, dp, flushed, ring_lo, lane);
+            }
             execute_group(w, w.stage + pr.delta, pr.g, dst, cap, dp, flushed, ring_lo, pr.done,
                           lane);
         } else {
@@ -1208,6 +1286,16 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                     r.kind = pr.kind;
                     r.status = pr.status;
                     r.done = pr.done ? 1u : 0u;
+                    r.lead = pr.lead ? 1u : 0u;
+                    if (pr.kind == kParseSlow || pr.lead) {
+                        r.lit_src = pr.q.lit_src;
+                        r.lit_dst = pr.q.lit_dst;
+                        r.lit_len_s = pr.q.lit_len;
+                        r.match_dst = pr.q.match_dst;
+                        r.offset_s = pr.q.offset;
+                        r.match_len_s = pr.q.match_len;
+                        r.dp_after = pr.lead ? pr.lead_dp : dp;
+                    }
                     if (pr.kind == kParseGroup) {
                         r.lit[lane] = (uint16_t)pr.g.lit;
                         r.lit_len[lane] = (uint16_t)pr.g.lit_len;
@@ -1218,14 +1306,6 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         r.take = pr.g.take;
                         r.terminal = pr.g.terminal ? 1u : 0u;
                         r.end_rel = pr.g.end_rel;
-                    } else if (pr.kind == kParseSlow) {
-                        r.lit_src = pr.q.lit_src;
-                        r.lit_dst = pr.q.lit_dst;
-                        r.lit_len_s = pr.q.lit_len;
-                        r.match_dst = pr.q.match_dst;
-                        r.offset_s = pr.q.offset;
-                        r.match_len_s = pr.q.match_len;
-                        r.dp_after = dp;
                     }
                     pdone = pr.kind == kParseFail || pr.done;
                     if (pdone) {
@@ -1237,29 +1317,32 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                 const uint32_t kind = uniform32(r.kind);
                 if (kind == kParseFail) {
                     status = (int)uniform32((uint32_t)r.status);
-                } else if (kind == kParseGroup) {
-                    Group g;
-                    g.lit = r.lit[lane];
-                    g.lit_len = r.lit_len[lane];
-                    g.match_len = r.match_len[lane];
-                    g.offset = r.offset[lane];
-                    g.drel = r.drel[lane];
-                    g.take = uniform32(r.take);
-                    g.terminal = uniform32(r.terminal) != 0;
-                    g.end_rel = uniform32(r.end_rel);
-                    const uint8_t* cw = (b ? L.w.stage : L.stage1) + uniform32(r.delta);
-                    execute_group(L.w, cw, g, dst, cap, dp, flushed, ring_lo,
-                                  uniform32(r.done) != 0, lane);
                 } else {
-                    Lz4Step s;
-                    s.lit_src = uniform64(r.lit_src);
-                    s.lit_dst = uniform64(r.lit_dst);
-                    s.lit_len = uniform64(r.lit_len_s);
-                    s.match_dst = uniform64(r.match_dst);
-                    s.offset = (uint32_t)uniform64(r.offset_s);
-                    s.match_len = uniform64(r.match_len_s);
-                    execute_slow(L.w, src, dst, cap, s, uniform64(r.dp_after), dp, flushed,
-                                 ring_lo, lane);
+                    if (kind == kParseSlow || uniform32(r.lead) != 0) {
+                        Lz4Step s;
+                        s.lit_src = uniform64(r.lit_src);
+                        s.lit_dst = uniform64(r.lit_dst);
+                        s.lit_len = uniform64(r.lit_len_s);
+                        s.match_dst = uniform64(r.match_dst);
+                        s.offset = (uint32_t)uniform64(r.offset_s);
+                        s.match_len = uniform64(r.match_len_s);
+                        execute_slow(L.w, src, dst, cap, s, uniform64(r.dp_after), dp, flushed,
+                                     ring_lo, lane);
+                    }
+                    if (kind == kParseGroup) {
+                        Group g;
+                        g.lit = r.lit[lane];
+                        g.lit_len = r.lit_len[lane];
+                        g.match_len = r.match_len[lane];
+                        g.offset = r.offset[lane];
+                        g.drel = r.drel[lane];
+                        g.take = uniform32(r.take);
+                        g.terminal = uniform32(r.terminal) != 0;
+                        g.end_rel = uniform32(r.end_rel);
+                        const uint8_t* cw = (b ? L.w.stage : L.stage1) + uniform32(r.delta);
+                        execute_group(L.w, cw, g, dst, cap, dp, flushed, ring_lo,
+                                      uniform32(r.done) != 0, lane);
+                    }
                 }
             }
             __syncthreads();
