@@ -35,6 +35,33 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
     return reinterpret_cast<T*>(uniform64(reinterpret_cast<uintptr_t>(p)));
 }
 
+/* Global-memory byte pointers. Chunk pointers arrive through tables, so the
+ * compiler sees generic pointers and would emit flat instructions, which
+ * count against the LDS wait counter too: every wait for an LDS read would
+ * also wait for outstanding global loads and stores. Casting them into the
+ * global address space (a no-op on the address) gives global_* instead. */
+typedef __attribute__((address_space(1))) uint8_t gu8;
+typedef __attribute__((address_space(1))) const uint8_t gcu8;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+
+/* A chunk pointer from a table, as a wave-uniform global pointer. */
+__device__ __forceinline__ gu8* global_bytes(void* p) {
+    return (gu8*)uniform64(reinterpret_cast<uintptr_t>(p));
+}
+__device__ __forceinline__ gcu8* global_bytes(const void* p) {
+    return (gcu8*)uniform64(reinterpret_cast<uintptr_t>(p));
+}
+
+/* 16 bytes from / to global memory at any alignment. */
+__device__ __forceinline__ uint4 gload16(gcu8* p) {
+    uint4 v;
+    __builtin_memcpy(&v, (const void*)p, 16);
+    return v;
+}
+__device__ __forceinline__ void gstore16(gu8* p, const uint4& v) {
+    __builtin_memcpy((void*)p, &v, 16);
+}
+
 /* Orders this wave's global-memory writes before its later reads of the
  * same bytes, across lanes. Within one wavefront the hardware issues and
  * completes vector memory operations in program order through one L1, so

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.h"
 #include "cudec.h"
 
 namespace cudec_amd {
@@ -34,7 +35,7 @@ constexpr uint64_t kLz4LitMargin = 15;  /* liblz4 ilimit, literal lengths */
 constexpr uint64_t kLz4MatchMargin = 4; /* liblz4 ilimit, match lengths */
 
 struct Lz4Cursor {
-    const uint8_t* src;
+    gcu8* src;
     uint64_t src_size;
     uint64_t cap;
     uint64_t sp;
@@ -155,7 +156,7 @@ __device__ __forceinline__ int lz4_next(Lz4Cursor& c, Lz4Step& q,
 
 /* Snappy (src/snappy_block.h:58-435). */
 struct SnappyCursor {
-    const uint8_t* src;
+    gcu8* src;
     uint64_t src_size;
     uint64_t cap;
     uint64_t sp;

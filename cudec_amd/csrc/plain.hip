@@ -17,7 +17,7 @@ constexpr unsigned kPlainBlock = 256;           /* 4 waves */
 constexpr unsigned kPlainMaxBlocks = 8192;
 
 /* A literal run followed by a match, in chunk-relative offsets. */
-__device__ __forceinline__ void plain_execute(const uint8_t* src, uint8_t* dst,
+__device__ __forceinline__ void plain_execute(gcu8* src, gu8* dst,
                                               uint32_t lane, uint64_t lit_src,
                                               uint64_t lit_dst,
                                               uint64_t lit_len,
@@ -60,8 +60,8 @@ __global__ void __launch_bounds__(kPlainBlock)
     uint64_t chunk = uniform64((uint64_t)blockIdx.x * waves_per_block +
                                threadIdx.x / kWave);
     for (; chunk < n; chunk += total_waves) {
-        const uint8_t* src = uniform_ptr(static_cast<const uint8_t*>(srcs[chunk]));
-        uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[chunk]));
+        gcu8* src = global_bytes(srcs[chunk]);
+        gu8* dst = global_bytes(dsts[chunk]);
         const uint64_t src_size = uniform64(src_sizes[chunk]);
         const uint64_t cap = uniform64(caps[chunk]);
         /* Fuel: every step consumes at least one source byte, so

@@ -92,7 +92,7 @@ __device__ __forceinline__ uint32_t ring_index(uintptr_t dst_addr, uint64_t pos)
  * chunk, so none leaves the chunk's pages); later blocks are zero-filled.
  * Bytes past src_size are never trusted: every member is validated against
  * src_size before anything it read is used. */
-__device__ __forceinline__ uint32_t stage_into(uint8_t* stage, const uint8_t* src,
+__device__ __forceinline__ uint32_t stage_into(uint8_t* stage, gcu8* src,
                                                uint64_t src_size, uint64_t sp,
                                                uint32_t lane) {
     const uintptr_t a = (uintptr_t)(src + sp);
@@ -104,7 +104,7 @@ __device__ __forceinline__ uint32_t stage_into(uint8_t* stage, const uint8_t* sr
         uint4 v = make_uint4(0, 0, 0, 0);
         /* only a block that holds at least one byte of [src, src_size) */
         if (g < end && g + 16 > lo) {
-            v = *reinterpret_cast<const uint4*>(g);
+            v = gload16((gcu8*)g);
         }
         *reinterpret_cast<uint4*>(&stage[16u * blk]) = v;
     }
@@ -172,7 +172,7 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane
 /* Stores ring bytes of positions [from, to) to dst. Full aligned 16-byte
  * blocks go out as one vector store per lane; the partial blocks at either
  * end (only at the chunk's first byte or its last) as single bytes. */
-__device__ __forceinline__ void flush_ring(const WaveLds& w, uint8_t* dst,
+__device__ __forceinline__ void flush_ring(const WaveLds& w, gu8* dst,
                                            uint64_t from, uint64_t to,
                                            uint32_t lane) {
     if (to <= from) {
@@ -194,16 +194,16 @@ __device__ __forceinline__ void flush_ring(const WaveLds& w, uint8_t* dst,
     const uint32_t tail = (uint32_t)(a1 - b1);
     if (lane < head) {
         const uintptr_t g = a0 + lane;
-        *reinterpret_cast<uint8_t*>(g) = w.ring[g & kRingMask];
+        *(gu8*)g = w.ring[g & kRingMask];
     } else if (lane >= 16 && lane < 16 + tail) {
         const uintptr_t g = b1 + (lane - 16);
-        *reinterpret_cast<uint8_t*>(g) = w.ring[g & kRingMask];
+        *(gu8*)g = w.ring[g & kRingMask];
     }
     const uint32_t blocks = (uint32_t)((b1 - b0) / 16);
     for (uint32_t b = lane; b < blocks; b += kWave) {
         const uintptr_t g = b0 + 16u * b;
         const uint4 v = *reinterpret_cast<const uint4*>(&w.ring[g & kRingMask]);
-        *reinterpret_cast<uint4*>(g) = v;
+        gstore16((gu8*)g, v);
     }
     wave_order();
 }
@@ -213,13 +213,13 @@ __device__ __forceinline__ void flush_ring(const WaveLds& w, uint8_t* dst,
  * global read takes the aligned 4-byte word holding the byte: a different
  * load width keeps the compiler from merging the two branches into one
  * flat load, and an aligned word never leaves the page of the byte. */
-__device__ __forceinline__ uint8_t history_byte(const WaveLds& w, const uint8_t* dst,
+__device__ __forceinline__ uint8_t history_byte(const WaveLds& w, gcu8* dst,
                                                 uint64_t y, uint64_t ring_lo) {
     if (y >= ring_lo) {
         return w.ring[ring_index((uintptr_t)dst, y)];
     }
     const uintptr_t a = (uintptr_t)(dst + y);
-    const uint32_t word = *reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+    const uint32_t word = *(gcu32*)(a & ~(uintptr_t)3);
     return (uint8_t)(word >> ((a & 3) * 8));
 }
 
@@ -280,7 +280,7 @@ __device__ __forceinline__ void ring_put(WaveLds& w, uint32_t r, const uint4& v,
  *  - otherwise the period is read bytewise.
  * Bytes past the period repeat it from the ring, 8 bytes a step from a
  * distance d that is a multiple of off and at least 8. */
-__device__ __forceinline__ void lane_match_copy(WaveLds& w, const uint8_t* dst, uintptr_t da,
+__device__ __forceinline__ void lane_match_copy(WaveLds& w, gcu8* dst, uintptr_t da,
                                                 uint64_t m, uint32_t off, uint32_t len,
                                                 uint64_t hist_lo, uint64_t cap) {
     const uint64_t y = m - off;
@@ -290,15 +290,15 @@ __device__ __forceinline__ void lane_match_copy(WaveLds& w, const uint8_t* dst, 
     if (y + need <= hist_lo && y + ((need + 15) & ~15u) <= cap) {
         uint4 h0, h1 = make_uint4(0, 0, 0, 0), h2 = make_uint4(0, 0, 0, 0),
                   h3 = make_uint4(0, 0, 0, 0);
-        __builtin_memcpy(&h0, dst + y, 16);
+        h0 = gload16(dst + y);
         if (need > 16) {
-            __builtin_memcpy(&h1, dst + y + 16, 16);
+            h1 = gload16(dst + y + 16);
         }
         if (need > 32) {
-            __builtin_memcpy(&h2, dst + y + 32, 16);
+            h2 = gload16(dst + y + 32);
         }
         if (need > 48) {
-            __builtin_memcpy(&h3, dst + y + 48, 16);
+            h3 = gload16(dst + y + 48);
         }
         ring_put(w, rd, h0, need < 16 ? need : 16);
         if (need > 16) {
@@ -347,7 +347,7 @@ __device__ __forceinline__ void lane_match_copy(WaveLds& w, const uint8_t* dst, 
 /* dst[0, n) = src[0, n), whole wave, global to global. Destination-aligned
  * 16-byte blocks; each lane's source read is an unaligned 16-byte load that
  * stays inside [src, src + n). */
-__device__ __forceinline__ void wave_copy_global(uint8_t* dst, const uint8_t* src,
+__device__ __forceinline__ void wave_copy_global(gu8* dst, gcu8* src,
                                                  uint64_t n, uint32_t lane) {
     if (n == 0) {
         return;
@@ -362,9 +362,7 @@ __device__ __forceinline__ void wave_copy_global(uint8_t* dst, const uint8_t* sr
     }
     const uint64_t body = (n - head) / 16;
     for (uint64_t b = lane; b < body; b += kWave) {
-        uint4 v;
-        __builtin_memcpy(&v, src + head + 16 * b, 16);
-        *reinterpret_cast<uint4*>(dst + head + 16 * b) = v;
+        gstore16(dst + head + 16 * b, gload16(src + head + 16 * b));
     }
     const uint64_t done = head + 16 * body;
     if (lane < n - done) {
@@ -387,7 +385,7 @@ static_assert(kStagedPeriod + 16 <= kRing, "a staged period fits the ring");
  *    byte-per-lane gather pays one per 64 bytes and whole-period rounds one
  *    per period;
  *  - otherwise: rounds of off (>= kStagedPeriod) bytes. */
-__device__ __forceinline__ void wave_match_global(WaveLds& w, uint8_t* dst, uint64_t m,
+__device__ __forceinline__ void wave_match_global(WaveLds& w, gu8* dst, uint64_t m,
                                                   uint64_t off, uint64_t len, uint64_t cap,
                                                   uint32_t lane) {
     if (off >= len) {
@@ -409,7 +407,7 @@ __device__ __forceinline__ void wave_match_global(WaveLds& w, uint8_t* dst, uint
     for (uint32_t b = 16 * lane; b < o; b += 16 * kWave) {
         uint4 v;
         if (from + b + 16 <= cap) {
-            __builtin_memcpy(&v, dst + from + b, 16);
+            v = gload16(dst + from + b);
         } else {
             uint32_t q[4] = {0, 0, 0, 0};
             for (uint32_t i = 0; i < 16 && b + i < o; i++) {
@@ -436,7 +434,7 @@ __device__ __forceinline__ void wave_match_global(WaveLds& w, uint8_t* dst, uint
         uint4 v;
         __builtin_memcpy(&v, &w.ring[r], 16);
         if (len - b >= 16) {
-            __builtin_memcpy(dst + m + b, &v, 16);
+            gstore16(dst + m + b, v);
         } else {
             const uint32_t n = (uint32_t)(len - b);
             for (uint32_t i = 0; i < n; i++) {
@@ -467,7 +465,7 @@ struct Group {
  * positions [dp, dp + g.end_rel), flushes whole 16-byte blocks (all of them
  * when the chunk is done) and advances dp. */
 __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, const Group& g,
-                                              uint8_t* dst, uint64_t cap, uint64_t& dp,
+                                              gu8* dst, uint64_t cap, uint64_t& dp,
                                               uint64_t& flushed, uint64_t ring_lo, bool done,
                                               uint32_t lane) {
     const uintptr_t da = (uintptr_t)dst;
@@ -577,7 +575,7 @@ struct Parsed {
 /* Stages the window at sp into `stage`, speculates, walks and resolves it
  * (src/lz4_block.h:95-258 for every member, in parallel) and advances
  * sp and dp past what it decided. Reads the source only. */
-__device__ __forceinline__ Parsed parse_window(uint8_t* stage, const uint8_t* src,
+__device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
                                                uint64_t src_size, uint64_t cap, uint64_t& sp,
                                                uint64_t& dp, uint32_t lane) {
     Parsed out;
@@ -852,7 +850,7 @@ __device__ __forceinline__ Cand parse_candidate_snappy(const uint8_t* cw, uint32
  * failing member in stream order decides), and the member whose position
  * is exactly the end of the source ends the chunk when the output so far
  * equals the declared length. */
-__device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, const uint8_t* src,
+__device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
                                                       uint64_t src_size, uint64_t cap,
                                                       uint64_t declared, uint64_t& sp,
                                                       uint64_t& dp, uint32_t lane) {
@@ -1031,7 +1029,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, const uint
 
 /* A window of either format. */
 template <int kFormat>
-__device__ __forceinline__ Parsed parse_any(uint8_t* stage, const uint8_t* src, uint64_t src_size,
+__device__ __forceinline__ Parsed parse_any(uint8_t* stage, gcu8* src, uint64_t src_size,
                                             uint64_t cap, uint64_t declared, uint64_t& sp,
                                             uint64_t& dp, uint32_t lane) {
     if (kFormat == kFormatSnappy) {
@@ -1043,7 +1041,7 @@ __device__ __forceinline__ Parsed parse_any(uint8_t* stage, const uint8_t* src, 
 /* Start of a chunk: the Snappy preamble and its declared length against the
  * capacity (src/snappy_block.h:190-263); nothing for LZ4. */
 template <int kFormat>
-__device__ __forceinline__ int chunk_begin(const uint8_t* src, uint64_t src_size, uint64_t cap,
+__device__ __forceinline__ int chunk_begin(gcu8* src, uint64_t src_size, uint64_t cap,
                                            uint64_t& sp, uint64_t& declared) {
     sp = 0;
     declared = 0;
@@ -1059,7 +1057,7 @@ __device__ __forceinline__ int chunk_begin(const uint8_t* src, uint64_t src_size
 
 /* Executes a sequence the window could not hold: everything in the ring
  * goes out first, then the literal and the match run global to global. */
-__device__ __forceinline__ void execute_slow(WaveLds& w, const uint8_t* src, uint8_t* dst,
+__device__ __forceinline__ void execute_slow(WaveLds& w, gcu8* src, gu8* dst,
                                              uint64_t cap, const Lz4Step& q, uint64_t dp_after,
                                              uint64_t& dp,
                                              uint64_t& flushed, uint64_t& ring_lo,
@@ -1117,8 +1115,8 @@ struct ChunkOut {
 
 /* One wave per chunk: parse a window, then execute it, in turn. */
 template <int kFormat>
-__device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, const uint8_t* src,
-                                                 uint64_t src_size, uint8_t* dst, uint64_t cap,
+__device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
+                                                 uint64_t src_size, gu8* dst, uint64_t cap,
                                                  uint32_t lane) {
     uint64_t sp = 0, declared = 0, pdp = 0, dp = 0, flushed = 0, ring_lo = 0;
     uint64_t fuel = src_size == UINT64_MAX ? UINT64_MAX : src_size + 2;
@@ -1251,8 +1249,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
             break;
         }
         const uint64_t chunk = order[idx];
-        const uint8_t* src = uniform_ptr(static_cast<const uint8_t*>(srcs[chunk]));
-        uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[chunk]));
+        gcu8* src = global_bytes(srcs[chunk]);
+        gu8* dst = global_bytes(dsts[chunk]);
         const uint64_t src_size = uniform64(src_sizes[chunk]);
         const uint64_t cap = uniform64(caps[chunk]);
         if ((int32_t)cost_bin(src_size, cap) > pipe_bin) {
@@ -1364,8 +1362,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
     uint32_t idx = parser ? split_idx : uniform32(atomicAdd(&q->next, lane == 0 ? 1u : 0u));
     while (idx < n) {
         const uint64_t chunk = order[idx];
-        const uint8_t* src = uniform_ptr(static_cast<const uint8_t*>(srcs[chunk]));
-        uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[chunk]));
+        gcu8* src = global_bytes(srcs[chunk]);
+        gu8* dst = global_bytes(dsts[chunk]);
         const uint64_t src_size = uniform64(src_sizes[chunk]);
         const uint64_t cap = uniform64(caps[chunk]);
         const ChunkOut o = decode_chunk<kFormat>(w, src, src_size, dst, cap, lane);
