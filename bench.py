@@ -57,6 +57,8 @@ def parse_args():
                    help="before timing, run a plain elementwise copy (torch.neg over int32) "
                         "of this many MiB three times: the known-size kernel that HBM "
                         "counters are scaled by")
+    p.add_argument("--no-graph", action="store_true",
+                   help="skip the HIP-graph-captured replay of the tuned launch")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default LZ4 pipelining threshold instead of tuning it "
                         "on the batch before timing")
@@ -116,16 +118,11 @@ def malform(b, pct, seed):
     return idx
 
 
-def time_form(db, fmt, form, steps, warmup, stream, orig_dev, batch, valid):
-    """Warmup, a byte-exact check of every intact chunk (status OK,
-    bytes_written = size, output = original), then `steps` event-timed
-    launches. Returns (wall seconds, per-launch ms, the checked results)."""
+def verify(db, form, orig_dev, batch, valid):
+    """Every intact chunk: status OK, bytes_written = size, output = original.
+    Returns the results read back."""
     import numpy as np
     import torch
-    for _ in range(warmup):
-        st = db.run(fmt, form, stream=stream)
-        assert st == 0, f"{form} submission failed: {st}"
-    torch.cuda.synchronize()
     res = db.results().copy()
     ok = bool(np.all(res["status"][valid] == 0)) and \
         bool(np.array_equal(res["bytes_written"][valid], batch.sizes[valid])) and \
@@ -145,6 +142,19 @@ def time_form(db, fmt, form, steps, warmup, stream, orig_dev, batch, valid):
         bad = int(np.sum(res["status"][valid] != 0))
         raise SystemExit(f"[bench] {form}: verification FAILED (intact non-OK chunks: {bad}, "
                          f"arena equal: {same})")
+    return res
+
+
+def time_form(db, fmt, form, steps, warmup, stream, orig_dev, batch, valid):
+    """Warmup, a byte-exact check of every intact chunk, then `steps`
+    event-timed launches. Returns (wall seconds, per-launch ms, the checked
+    results)."""
+    import torch
+    for _ in range(warmup):
+        st = db.run(fmt, form, stream=stream)
+        assert st == 0, f"{form} submission failed: {st}"
+    torch.cuda.synchronize()
+    res = verify(db, form, orig_dev, batch, valid)
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     torch.cuda.synchronize()
@@ -160,7 +170,44 @@ def time_form(db, fmt, form, steps, warmup, stream, orig_dev, batch, valid):
     return wall, kernel_ms, res
 
 
-def cpu_baseline(batch, fmt, threads, valid):
+def graph_leg(db, fmt, steps, stream, orig_dev, batch, valid, eager):
+    """The tuned launch captured once into a HIP graph (cudec_amd/graph.py)
+    and replayed `steps` times, event-timed; the output is cleared before
+    the replays and checked after them, and the results must equal the
+    eager launch's."""
+    import numpy as np
+    import torch
+    from cudec_amd.graph import CapturedLaunch
+    with CapturedLaunch(lambda: db.run(fmt, "tuned", stream=stream), stream) as g:
+        with torch.cuda.stream(stream):
+            db.d_out.zero_()
+            db.reset_results()
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        for k in range(steps):
+            starts[k].record(stream)
+            g.replay()
+            ends[k].record(stream)
+        torch.cuda.synchronize()
+    res = verify(db, "graph", orig_dev, batch, valid)
+    if not np.array_equal(res, eager):
+        raise SystemExit("[bench] graph replay results differ from the eager launch")
+    ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return {"replays": steps, "avg_kernel_ms": round(sum(ms) / len(ms), 4), "verified": True}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(batch, fmt, threads, valid, runs=30):
     """The reference ladder compiled from its own source (oracle/_ref) when
     that build is present, else the C oracle: `threads` host threads, one
     pass over the whole batch into pre-sized buffers, decode calls only.
@@ -203,24 +250,68 @@ def cpu_baseline(batch, fmt, threads, valid):
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
         t0 = time.perf_counter()
         list(ex.map(work, range(threads)))
-        secs = time.perf_counter() - t0
-    intact_ok = bool(np.all(status[valid] == 0)) and \
-        bool(np.array_equal(written[valid], batch.sizes[valid]))
-    if intact_ok and bool(np.all(valid)):
-        intact_ok = bool(np.array_equal(out, batch.orig))
-    elif intact_ok:
-        for i in np.nonzero(valid)[0]:
-            o, m = int(batch.orig_off[i]), int(batch.sizes[i])
-            if not np.array_equal(out[o:o + m], batch.orig[o:o + m]):
-                intact_ok = False
-                break
-    if not intact_ok:
-        raise SystemExit("[bench] CPU baseline produced wrong output")
-    rec = {"value": round(batch.total_uncompressed / secs / 1e9, 3), "unit": "GB/s (uncompressed)",
+        first = time.perf_counter() - t0
+        intact_ok = bool(np.all(status[valid] == 0)) and \
+            bool(np.array_equal(written[valid], batch.sizes[valid]))
+        if intact_ok and bool(np.all(valid)):
+            intact_ok = bool(np.array_equal(out, batch.orig))
+        elif intact_ok:
+            for i in np.nonzero(valid)[0]:
+                o, m = int(batch.orig_off[i]), int(batch.sizes[i])
+                if not np.array_equal(out[o:o + m], batch.orig[o:o + m]):
+                    intact_ok = False
+                    break
+        if not intact_ok:
+            raise SystemExit("[bench] CPU baseline produced wrong output")
+        # the checked pass is the warmup; then timed passes, bounded to about
+        # 10 s of wall time
+        runs = max(1, min(runs, int(10.0 / max(first, 1e-3))))
+        secs = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            list(ex.map(work, range(threads)))
+            secs.append(time.perf_counter() - t0)
+    from cudec_amd.serve import percentile
+
+    def rate(t, nbytes):
+        return round(nbytes / t / 1e9, 3)
+    # one thread over a sample: the batch's first chunks (the batch order is
+    # shuffled) up to 256 MiB of output, 3 warmup + 30 timed passes
+    m = int(np.searchsorted(np.cumsum(batch.sizes.astype(np.int64)), 256 << 20, side="right"))
+    m = max(1, min(m, n))
+    sample_bytes = int(batch.sizes[:m].sum())
+    single_bounds = (0, m)
+
+    def single():
+        w = u64(0)
+        for i in range(*single_bounds):
+            args = [cbase + int(batch.comp_off[i]), int(batch.comp_size[i]),
+                    obase + int(batch.orig_off[i]), int(batch.sizes[i]), ctypes.byref(w)]
+            if extra:
+                args.append(None)
+            fn(*args)
+
+    one = []
+    for k in range(33):
+        t0 = time.perf_counter()
+        single()
+        if k >= 3:
+            one.append(time.perf_counter() - t0)
+    total = batch.total_uncompressed
+    # rates at the nearest-rank p50 / p90 / p99 pass times
+    rec = {"value": rate(percentile(secs, 50), total), "unit": "GB/s (uncompressed)",
            "cores": threads, "kind": kind,
-           "sample": f"one pass over the full batch ({n} chunks, "
-                     f"{batch.total_uncompressed / 2**30:.2f} GiB out), {threads} threads, "
-                     f"{secs:.2f} s wall"}
+           "sample": f"the full batch ({n} chunks, {batch.total_uncompressed / 2**30:.2f} GiB "
+                     f"out), {threads} threads, static contiguous partition; 1 checked warmup "
+                     f"pass + {len(secs)} timed passes, nearest-rank p50 "
+                     f"({min(secs):.2f}-{max(secs):.2f} s wall per pass)",
+           "at_p90_time": rate(percentile(secs, 90), total),
+           "at_p99_time": rate(percentile(secs, 99), total),
+           "single_thread": {"value": rate(percentile(one, 50), sample_bytes),
+                             "at_p90_time": rate(percentile(one, 90), sample_bytes),
+                             "sample": f"first {m} chunks ({sample_bytes / 2**20:.0f} MiB out), "
+                                       "1 thread, 3 warmup + 30 timed passes, nearest-rank p50"},
+           "cpu_model": cpu_model()}
     return rec, status, written
 
 
@@ -367,6 +458,11 @@ def main():
             f"uncompressed (wall), {measured[form]['algo_gbps']:.1f} GB/s algorithmic "
             f"= {100 * measured[form]['algo_gbps'] / HBM_PEAK_GBPS:.1f}% of HBM peak")
     head = measured[forms[0]]
+    graph = None
+    if "tuned" in measured and not args.no_graph:
+        graph = graph_leg(db, args.format, args.steps, stream, orig_dev, batch, valid,
+                          measured["tuned"]["results"])
+        log(f"graph: {graph['avg_kernel_ms']:.3f} ms/replay over {graph['replays']} replays, verified")
     cpu = None
     malformed_checked = False
     if not args.no_cpu:
@@ -419,6 +515,8 @@ def main():
                          "algo_gbps": round(measured["plain"]["algo_gbps"], 2)}
         line["speedup_vs_plain"] = round(measured["plain"]["avg_kernel_ms"] / head["avg_kernel_ms"], 3)
     line["avg_kernel_ms"] = round(head["avg_kernel_ms"], 4)
+    if graph is not None:
+        line["graph"] = graph
     if tuning is not None:
         line["autotune"] = tuning
     if len(malformed):

@@ -1490,9 +1490,17 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
         min_cost = dev->lz4_pipe_min_cost;
     }
     const size_t bytes = sizeof(Queue) + sizeof(uint32_t) * n;
+    /* under stream capture the scratch becomes a graph allocation node of
+     * the device's default pool; eager launches use the keeping pool */
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    e = hipStreamIsCapturing(stream, &capture);
+    if (e != hipSuccess) {
+        return e;
+    }
     void* scratch = nullptr;
-    e = dev->pool != nullptr ? hipMallocFromPoolAsync(&scratch, bytes, dev->pool, stream)
-                             : hipMallocAsync(&scratch, bytes, stream);
+    e = dev->pool != nullptr && capture == hipStreamCaptureStatusNone
+            ? hipMallocFromPoolAsync(&scratch, bytes, dev->pool, stream)
+            : hipMallocAsync(&scratch, bytes, stream);
     if (e != hipSuccess) {
         return e;
     }
