@@ -555,6 +555,70 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
 
 /* ------------------------------------------------- window parse */
 
+/* Follows the links from `start` (a scalar chain: each step reads the next
+ * position out of the lane that holds the current one). Lane j of `member`
+ * receives member j's position. The walk ends at a stop link (that position
+ * is not a member), past the window, or after 64 members; it returns the
+ * position after the last member and the member count. */
+__device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kCand], uint32_t start,
+                                               uint32_t& count_out, uint32_t& member) {
+    static_assert(kStop >= kCand * kWave + kWave, "a stop link must leave every region");
+    uint32_t pos = start, count = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kCand; k++) {
+        uint32_t rel = pos - k * kWave;
+        if ((rel | count) >= kWave) {
+            if (count == kWave) {
+                break;
+            }
+            continue; /* pos lies in a later region */
+        }
+        /* Per member: read the link of pos out of lane rel, record pos as
+         * member `count`, step. The loop leaves once pos is outside this
+         * region (a stop link lands far outside) or 64 members are held.
+         * Only scalar registers are written apart from `member`. */
+        uint32_t prev, nx, t;
+        /* The count lives in m0 during the loop (saved and restored around
+         * it): v_writelane takes its value and lane select from scalar
+         * registers, and only m0 as the lane select keeps it within one
+         * constant-bus read. The leading s_nop covers a lane select (rel)
+         * that the compiler may have just written from a vector
+         * instruction, and m0 written just before its first use. */
+        uint32_t m0_saved;
+        asm volatile(
+            "s_mov_b32 %[m0s], m0\n\t"
+            "s_mov_b32 m0, %[cnt]\n\t"
+            "s_nop 4\n"
+            "walk_loop_%=:\n\t"
+            "v_readlane_b32 %[nx], %[lk], %[rel]\n\t"
+            "v_writelane_b32 %[mem], %[pos], m0\n\t"
+            "s_mov_b32 %[prev], %[pos]\n\t"
+            "s_mov_b32 %[pos], %[nx]\n\t"
+            "s_sub_u32 %[rel], %[nx], %[base]\n\t"
+            "s_add_u32 m0, m0, 1\n\t"
+            "s_or_b32 %[t], %[rel], m0\n\t"
+            "s_cmp_lt_u32 %[t], 64\n\t"
+            "s_cbranch_scc1 walk_loop_%=\n\t"
+            "s_mov_b32 %[cnt], m0\n\t"
+            "s_mov_b32 m0, %[m0s]"
+            : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+s"(count), [rel] "+s"(rel),
+              [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [m0s] "=&s"(m0_saved)
+            : [lk] "v"(link[k]), [base] "s"(k * kWave)
+            : "scc");
+        if (pos == kStop) {
+            /* prev has a stop link: it is not a member, and the walk ends there */
+            pos = prev;
+            count--;
+            break;
+        }
+        if (count == kWave) {
+            break;
+        }
+    }
+    count_out = uniform32(count);
+    return uniform32(pos);
+}
+
 /* What parsing one window decided. kGroup: execute `g` (staged at `delta`
  * in the stage buffer) and advance the source by `consumed`. kSlow: one
  * sequence the window cannot hold, by the scalar ladder (`q`, with the
@@ -623,24 +687,9 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
     }
 
     /* walk the true chain */
-    uint32_t pos = start, count = 0;
+    uint32_t count = 0;
     uint32_t member = 0; /* lane j: position of member j */
-    bool stopped = false;
-#pragma unroll
-    for (uint32_t k = 0; k < kCand; k++) {
-        while (!stopped && pos < (k + 1) * kWave && count < kWave) {
-            const uint32_t nx = __builtin_amdgcn_readlane(link[k], pos - k * kWave);
-            if (nx == kStop) {
-                stopped = true;
-                break;
-            }
-            member = lane == count ? pos : member;
-            count++;
-            pos = nx;
-        }
-    }
-    count = uniform32(count);
-    pos = uniform32(pos);
+    const uint32_t pos = walk_chain(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
@@ -901,24 +950,9 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
         }
         start = (uint32_t)adv;
     }
-    uint32_t pos = start, count = 0;
+    uint32_t count = 0;
     uint32_t member = 0;
-    bool stopped = false;
-#pragma unroll
-    for (uint32_t k = 0; k < kCand; k++) {
-        while (!stopped && pos < (k + 1) * kWave && count < kWave) {
-            const uint32_t nx = __builtin_amdgcn_readlane(link[k], pos - k * kWave);
-            if (nx == kStop) {
-                stopped = true;
-                break;
-            }
-            member = lane == count ? pos : member;
-            count++;
-            pos = nx;
-        }
-    }
-    count = uniform32(count);
-    pos = uniform32(pos);
+    const uint32_t pos = walk_chain(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
