@@ -7,12 +7,13 @@
  * of its block, as a software pipeline. Once the queue reaches cheaper
  * chunks, a block splits into two independent one-wave decoders.
  *
- * Per chunk, the stream is decoded in windows of kWin candidate positions
+ * Per chunk, the stream is decoded in windows of candidate positions
  * staged in LDS:
  *   1. stage     -- compressed bytes from the window start are loaded as
  *                   aligned 16-byte blocks (a block holding any byte of the
  *                   chunk never leaves the chunk's pages);
- *   2. speculate -- every lane parses 8 candidate positions as if a sequence
+ *   2. speculate -- every lane parses 16 (two-wave form) or 8 (one-wave
+ *                   form) candidate positions as if a sequence
  *                   (LZ4) or element (Snappy) started there, and records
  *                   where the next one would start;
  *   3. walk      -- a scalar walk follows the true chain from position 0
@@ -47,10 +48,17 @@
 namespace cudec_amd {
 namespace {
 
-constexpr uint32_t kWin = 512;            /* candidate positions per window */
-constexpr uint32_t kCand = kWin / kWave;  /* candidates per lane */
-constexpr uint32_t kStage = 1024;         /* staged bytes used by a window */
-constexpr uint32_t kStageBuf = kStage + 16;
+/* Candidates per lane: a window spans kC * 64 candidate positions. The
+ * two-wave form parses windows twice as wide as the one-wave form: its
+ * parser runs beside the copier, and a wider window fills more lanes of
+ * every lane-parallel step, while a lone wave pays the wider speculation
+ * on its own critical path. */
+constexpr uint32_t kCandSplit = 8;
+constexpr uint32_t kCandPipe = 16;
+/* staged bytes a window of kC candidates per lane reads (the window plus
+ * the longest candidate that starts in it) */
+constexpr uint32_t stage_bytes(uint32_t kc) { return kc * 64u + 512u; }
+constexpr uint32_t kStageBuf = stage_bytes(kCandPipe) + 16;
 constexpr uint32_t kRing = 4096;          /* per-wave output ring */
 constexpr uint32_t kRingMask = kRing - 1;
 constexpr uint32_t kBatchMax = 2048;      /* output bytes of one batch */
@@ -64,7 +72,9 @@ constexpr uint64_t kDefaultPipeMinCost = 512u << 10;
 
 static_assert(kBatchMax + 32 <= kRing, "ring must hold a batch and a margin");
 static_assert(kLaneMatch <= 64, "a lane match copies at most four 16-byte blocks");
-static_assert(kWin + 2 + 269 + 3 <= kStage, "a window's last candidate fits the stage");
+static_assert(kCandSplit * 64 + 2 + 269 + 3 <= stage_bytes(kCandSplit) &&
+                  kCandPipe * 64 + 2 + 269 + 3 <= stage_bytes(kCandPipe),
+              "a window's last candidate fits the stage");
 
 struct alignas(16) WaveLds {
     uint8_t ring[kRing];
@@ -86,12 +96,13 @@ __device__ __forceinline__ uint32_t ring_index(uintptr_t dst_addr, uint64_t pos)
 
 /* ------------------------------------------------------------ staging */
 
-/* Loads compressed bytes [sp, sp + kStage) of the chunk into lds.stage and
+/* Loads compressed bytes [sp, sp + stage_bytes(kC)) of the chunk into stage and
  * returns the offset of byte sp inside it. Only aligned 16-byte blocks that
  * start before the chunk end are read (each holds at least one byte of the
  * chunk, so none leaves the chunk's pages); later blocks are zero-filled.
  * Bytes past src_size are never trusted: every member is validated against
  * src_size before anything it read is used. */
+template <uint32_t kC>
 __device__ __forceinline__ uint32_t stage_into(uint8_t* stage, gcu8* src,
                                                uint64_t src_size, uint64_t sp,
                                                uint32_t lane) {
@@ -99,7 +110,7 @@ __device__ __forceinline__ uint32_t stage_into(uint8_t* stage, gcu8* src,
     const uintptr_t base = a & ~(uintptr_t)15;
     const uintptr_t lo = (uintptr_t)src;
     const uintptr_t end = lo + src_size;
-    for (uint32_t blk = lane; blk < kStageBuf / 16; blk += kWave) {
+    for (uint32_t blk = lane; blk < stage_bytes(kC) / 16 + 1; blk += kWave) {
         const uintptr_t g = base + 16u * blk;
         uint4 v = make_uint4(0, 0, 0, 0);
         /* only a block that holds at least one byte of [src, src_size) */
@@ -125,7 +136,7 @@ struct Cand {
 };
 
 /* The sequence a token at window position c would encode, assuming at most
- * one extension byte per length. Reads stay below c + 275 < kStage. */
+ * one extension byte per length. Reads stay below c + 275, inside the stage. */
 __device__ __forceinline__ Cand parse_candidate(const uint8_t* cw, uint32_t c) {
     Cand k;
     k.tok = cw[c];
@@ -560,12 +571,13 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
  * receives member j's position. The walk ends at a stop link (that position
  * is not a member), past the window, or after 64 members; it returns the
  * position after the last member and the member count. */
-__device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kCand], uint32_t start,
+template <uint32_t kC>
+__device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint32_t start,
                                                uint32_t& count_out, uint32_t& member) {
-    static_assert(kStop >= kCand * kWave + kWave, "a stop link must leave every region");
+    static_assert(kStop >= kC * kWave + kWave, "a stop link must leave every region");
     uint32_t pos = start, count = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kCand; k++) {
+    for (uint32_t k = 0; k < kC; k++) {
         uint32_t rel = pos - k * kWave;
         if ((rel | count) >= kWave) {
             if (count == kWave) {
@@ -639,6 +651,7 @@ struct Parsed {
 /* Stages the window at sp into `stage`, speculates, walks and resolves it
  * (src/lz4_block.h:95-258 for every member, in parallel) and advances
  * sp and dp past what it decided. Reads the source only. */
+template <uint32_t kC>
 __device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
                                                uint64_t src_size, uint64_t cap, uint64_t& sp,
                                                uint64_t& dp, uint32_t lane) {
@@ -649,13 +662,13 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
     out.lead = false;
     out.lead_dp = 0;
     const uint64_t base = sp; /* window start; member positions are relative to it */
-    out.delta = stage_into(stage, src, src_size, sp, lane);
+    out.delta = stage_into<kC>(stage, src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
 
     /* speculate: the link of each candidate position */
-    uint32_t link[kCand];
+    uint32_t link[kC];
 #pragma unroll
-    for (uint32_t k = 0; k < kCand; k++) {
+    for (uint32_t k = 0; k < kC; k++) {
         const Cand c = parse_candidate(cw, k * kWave + lane);
         link[k] = c.complex ? kStop : c.next;
     }
@@ -677,7 +690,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
         out.lead = true;
         out.lead_dp = dp;
         const uint64_t adv = sp - base;
-        if (uniform32(last) != 0 || adv >= kWin) {
+        if (uniform32(last) != 0 || adv >= kC * kWave) {
             out.kind = kParseSlow;
             out.lead = false;
             out.done = uniform32(last) != 0;
@@ -689,7 +702,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
     /* walk the true chain */
     uint32_t count = 0;
     uint32_t member = 0; /* lane j: position of member j */
-    const uint32_t pos = walk_chain(link, start, count, member);
+    const uint32_t pos = walk_chain<kC>(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
@@ -840,9 +853,11 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
 /* ------------------------------------------------- Snappy window parse */
 
 /* Longest literal a Snappy window resolves; longer ones take the scalar
- * ladder. A window literal therefore ends below 512 + 5 + 256. */
+ * ladder. A window literal therefore ends below kC * 64 + 5 + 256. */
 constexpr uint32_t kSnappyWinLit = 256;
-static_assert(kWin + 5 + kSnappyWinLit + 16 <= kStageBuf, "stage covers a Snappy window");
+static_assert(kCandSplit * 64 + 5 + kSnappyWinLit + 16 <= stage_bytes(kCandSplit) + 16 &&
+                  kCandPipe * 64 + 5 + kSnappyWinLit + 16 <= stage_bytes(kCandPipe) + 16,
+              "stage covers a Snappy window");
 
 /* The element a tag at window position c would encode
  * (src/snappy_block.h:288-435): a literal (lit, lit_len; match_len 0) or a
@@ -899,6 +914,7 @@ __device__ __forceinline__ Cand parse_candidate_snappy(const uint8_t* cw, uint32
  * failing member in stream order decides), and the member whose position
  * is exactly the end of the source ends the chunk when the output so far
  * equals the declared length. */
+template <uint32_t kC>
 __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
                                                       uint64_t src_size, uint64_t cap,
                                                       uint64_t declared, uint64_t& sp,
@@ -910,12 +926,12 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
     out.lead = false;
     out.lead_dp = 0;
     const uint64_t base = sp;
-    out.delta = stage_into(stage, src, src_size, sp, lane);
+    out.delta = stage_into<kC>(stage, src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
 
-    uint32_t link[kCand];
+    uint32_t link[kC];
 #pragma unroll
-    for (uint32_t k = 0; k < kCand; k++) {
+    for (uint32_t k = 0; k < kC; k++) {
         const Cand c = parse_candidate_snappy(cw, k * kWave + lane);
         link[k] = c.complex ? kStop : c.next;
     }
@@ -942,7 +958,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
         out.lead = true;
         out.lead_dp = dp;
         const uint64_t adv = sp - base;
-        if (uniform32(last) != 0 || adv >= kWin) {
+        if (uniform32(last) != 0 || adv >= kC * kWave) {
             out.kind = kParseSlow;
             out.lead = false;
             out.done = uniform32(last) != 0;
@@ -952,7 +968,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
     }
     uint32_t count = 0;
     uint32_t member = 0;
-    const uint32_t pos = walk_chain(link, start, count, member);
+    const uint32_t pos = walk_chain<kC>(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
@@ -1062,14 +1078,14 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
 }
 
 /* A window of either format. */
-template <int kFormat>
+template <int kFormat, uint32_t kC>
 __device__ __forceinline__ Parsed parse_any(uint8_t* stage, gcu8* src, uint64_t src_size,
                                             uint64_t cap, uint64_t declared, uint64_t& sp,
                                             uint64_t& dp, uint32_t lane) {
     if (kFormat == kFormatSnappy) {
-        return parse_window_snappy(stage, src, src_size, cap, declared, sp, dp, lane);
+        return parse_window_snappy<kC>(stage, src, src_size, cap, declared, sp, dp, lane);
     }
-    return parse_window(stage, src, src_size, cap, sp, dp, lane);
+    return parse_window<kC>(stage, src, src_size, cap, sp, dp, lane);
 }
 
 /* Start of a chunk: the Snappy preamble and its declared length against the
@@ -1160,7 +1176,8 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
             status = kCorrupt;
             break;
         }
-        const Parsed pr = parse_any<kFormat>(w.stage, src, src_size, cap, declared, sp, pdp, lane);
+        const Parsed pr =
+            parse_any<kFormat, kCandSplit>(w.stage, src, src_size, cap, declared, sp, pdp, lane);
         if (pr.kind == kParseFail) {
             status = pr.status;
             break;
@@ -1312,7 +1329,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         pr.done = false;
                         pr.delta = 0;
                     } else {
-                        pr = parse_any<kFormat>(b ? L.stage1 : L.w.stage, src, src_size, cap,
+                        pr = parse_any<kFormat, kCandPipe>(b ? L.stage1 : L.w.stage, src,
+                                                           src_size, cap,
                                                 declared, sp, dp, lane);
                     }
                     r.kind = pr.kind;
