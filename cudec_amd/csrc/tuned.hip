@@ -547,9 +547,28 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
             if (!jdone) {
                 const uint64_t jm0 = jd0 + jl;
                 const uint64_t jfrom = jm0 - joff;
-                for (uint32_t i = lane; i < jm; i += kWave) {
-                    const uint64_t yy = jfrom + (i < joff ? i : i % joff);
-                    w.ring[ring_index(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
+                if (jm <= kWave && jfrom >= hist_lo) {
+                    /* the common case: a short match whose period is in the
+                     * ring, a byte a lane. Byte i copies period byte
+                     * i mod joff (i < 64; the float quotient is exact after
+                     * one correction), which precedes the match, so the
+                     * lanes do not depend on each other. */
+                    const uint32_t rs = ring_index(da, jfrom);
+                    const uint32_t rm = ring_index(da, jm0);
+                    uint32_t r = lane;
+                    if (lane >= joff) {
+                        const float inv = __builtin_amdgcn_rcpf((float)joff);
+                        r = lane - joff * (uint32_t)((float)lane * inv);
+                        r = r >= joff ? r - joff : r;
+                    }
+                    if (lane < jm) {
+                        w.ring[(rm + lane) & kRingMask] = w.ring[(rs + r) & kRingMask];
+                    }
+                } else {
+                    for (uint32_t i = lane; i < jm; i += kWave) {
+                        const uint64_t yy = jfrom + (i < joff ? i : i % joff);
+                        w.ring[ring_index(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
+                    }
                 }
             }
         }
