@@ -281,35 +281,98 @@ __device__ __forceinline__ void ring_put(WaveLds& w, uint32_t r, const uint4& v,
     }
 }
 
+/* Four selector bytes (r + i) mod p, i = 0..3, packed, for 3 <= p <= 15 and
+ * r < p, given pb = p and cp = 0x80 - p in every byte. A bytewise compare
+ * and subtract in one word: every byte stays below 128, so no carry or
+ * borrow crosses a byte, and ge * 255 is (ge << 8) - ge modulo 2^32. */
+__device__ __forceinline__ uint32_t period_sel(uint32_t r, uint32_t pb, uint32_t cp) {
+    const uint32_t sv = __builtin_amdgcn_perm(0u, r, 0u) + 0x03020100u;
+    const uint32_t ge = ((sv + cp) >> 7) & 0x01010101u;
+    return sv - (((ge << 8) - ge) & pb);
+}
+
+/* Bytes of the 16-byte block S picked by four selector bytes (each 0..15). */
+__device__ __forceinline__ uint32_t gather16(const uint4& S, uint32_t sel) {
+    const uint32_t lo = __builtin_amdgcn_perm(S.y, S.x, sel & 0x07070707u);
+    const uint32_t hi = __builtin_amdgcn_perm(S.w, S.z, sel & 0x07070707u);
+    const uint32_t pick = 0x03020100u + ((sel >> 1) & 0x04040404u);
+    return __builtin_amdgcn_perm(hi, lo, pick);
+}
+
+/* The source period of a lane match lies wholly in flushed history, so its
+ * bytes can be loaded before the window's literals are placed. */
+__device__ __forceinline__ bool period_in_history(uint64_t m, uint32_t off, uint32_t len,
+                                                  uint64_t hist_lo, uint64_t cap) {
+    const uint64_t y = m - off;
+    const uint32_t need = len < off ? len : off;
+    return y + need <= hist_lo && y + ((need + 15) & ~15u) <= cap;
+}
+
+/* Up to four 16-byte global loads of a history period, issued together. */
+struct Period {
+    uint4 h0, h1, h2, h3;
+};
+__device__ __forceinline__ Period load_period(gcu8* dst, uint64_t y, uint32_t need) {
+    Period p;
+    p.h1 = make_uint4(0, 0, 0, 0);
+    p.h2 = make_uint4(0, 0, 0, 0);
+    p.h3 = make_uint4(0, 0, 0, 0);
+    p.h0 = gload16(dst + y);
+    if (need > 16) {
+        p.h1 = gload16(dst + y + 16);
+    }
+    if (need > 32) {
+        p.h2 = gload16(dst + y + 32);
+    }
+    if (need > 48) {
+        p.h3 = gload16(dst + y + 48);
+    }
+    return p;
+}
+
 /* One lane copies a match of len (<= kLaneMatch) bytes at output position
  * m with offset off into the ring; every source byte precedes m or is
- * written earlier by this lane.
- *  - the source period [m - off, m - off + min(len, off)) lies wholly in
- *    flushed history: up to four 16-byte global loads, issued together, so
- *    the whole wave pays one round trip instead of one per byte;
- *  - it lies wholly in the ring with off >= 8: 8-byte ring steps;
- *  - otherwise the period is read bytewise.
- * Bytes past the period repeat it from the ring, 8 bytes a step from a
- * distance d that is a multiple of off and at least 8. */
+ * written earlier by this lane. Two steps:
+ *  1. the period S = [m - off, m - off + min(len, off)) goes to the ring at
+ *     m: from flushed history (from_history: hp holds it, loaded by the
+ *     caller with up to four 16-byte loads issued together), or from the
+ *     ring with independent 16-byte reads, or bytewise where it straddles
+ *     the ring's floor or wraps;
+ *  2. an overlapping match (off < len) repeats it: for off < 16 the first
+ *     16 bytes are built in registers from S (byte k = S[k mod off]) and
+ *     the rest is copied from D = off * floor(16 / off) bytes back, 8 or 16
+ *     bytes a step; for off >= 16, 16 bytes a step from off back. Where the
+ *     match wraps the ring, bytewise. */
 __device__ __forceinline__ void lane_match_copy(WaveLds& w, gcu8* dst, uintptr_t da,
                                                 uint64_t m, uint32_t off, uint32_t len,
-                                                uint64_t hist_lo, uint64_t cap) {
+                                                uint64_t hist_lo, uint64_t cap, bool from_history,
+                                                const Period& hp) {
     const uint64_t y = m - off;
     const uint32_t need = len < off ? len : off;
     const uint32_t rd = ring_index(da, m);
-    uint32_t i = 0;
-    if (y + need <= hist_lo && y + ((need + 15) & ~15u) <= cap) {
-        uint4 h0, h1 = make_uint4(0, 0, 0, 0), h2 = make_uint4(0, 0, 0, 0),
-                  h3 = make_uint4(0, 0, 0, 0);
-        h0 = gload16(dst + y);
-        if (need > 16) {
-            h1 = gload16(dst + y + 16);
-        }
-        if (need > 32) {
-            h2 = gload16(dst + y + 32);
-        }
-        if (need > 48) {
-            h3 = gload16(dst + y + 48);
+    const uint32_t ry = ring_index(da, y);
+    /* step 1 */
+    if (from_history || (y >= hist_lo && ry + need <= kRing)) {
+        uint4 h0, h1, h2, h3;
+        if (from_history) {
+            h0 = hp.h0;
+            h1 = hp.h1;
+            h2 = hp.h2;
+            h3 = hp.h3;
+        } else {
+            /* reads past `need` (at most 15 bytes, still inside this
+             * WaveLds) are never stored */
+            __builtin_memcpy(&h0, &w.ring[ry], 16);
+            h1 = h2 = h3 = make_uint4(0, 0, 0, 0);
+            if (need > 16) {
+                __builtin_memcpy(&h1, &w.ring[ry + 16], 16);
+            }
+            if (need > 32) {
+                __builtin_memcpy(&h2, &w.ring[ry + 32], 16);
+            }
+            if (need > 48) {
+                __builtin_memcpy(&h3, &w.ring[ry + 48], 16);
+            }
         }
         ring_put(w, rd, h0, need < 16 ? need : 16);
         if (need > 16) {
@@ -321,33 +384,59 @@ __device__ __forceinline__ void lane_match_copy(WaveLds& w, gcu8* dst, uintptr_t
         if (need > 48) {
             ring_put(w, (rd + 48) & kRingMask, h3, need - 48);
         }
-        i = need;
-    } else if (y >= hist_lo && rd >= off && rd + len <= kRing && off >= 8) {
-        for (; i + 8 <= len; i += 8) {
-            uint64_t v;
-            __builtin_memcpy(&v, &w.ring[rd - off + i], 8);
-            __builtin_memcpy(&w.ring[rd + i], &v, 8);
-        }
-    } else if (y < hist_lo) {
-        for (; i < need; i++) {
+    } else {
+        for (uint32_t i = 0; i < need; i++) {
             w.ring[(rd + i) & kRingMask] = history_byte(w, dst, y + i, hist_lo);
         }
     }
-    if (i < len) {
-        /* d for off = 1..7 is 8, 8, 9, 8, 10, 12, 14 */
-        const uint32_t d =
-            off >= 8 ? off : (uint32_t)((0x0E0C0A08090808ull >> (8 * (off - 1))) & 0xFF);
-        for (; i < len && i < d; i++) {
-            w.ring[(rd + i) & kRingMask] = w.ring[(rd + i - off) & kRingMask];
+    if (need == len) {
+        return;
+    }
+    /* step 2: off < len */
+    if (off < 16 && rd + (len > 16 ? len : 16) <= kRing) {
+        uint4 S;
+        __builtin_memcpy(&S, &w.ring[rd], 16);
+        uint32_t p = off;
+        if (p <= 2) {
+            /* periods 1 and 2 as period 4 */
+            S.x = __builtin_amdgcn_perm(0u, S.x, p == 1 ? 0x00000000u : 0x01000100u);
+            p = 4;
         }
-        if (rd + len <= kRing) {
-            for (; i + 8 <= len; i += 8) {
-                uint64_t v;
-                __builtin_memcpy(&v, &w.ring[rd + i - d], 8);
-                __builtin_memcpy(&w.ring[rd + i], &v, 8);
+        const uint32_t pb = __builtin_amdgcn_perm(0u, p, 0u);
+        const uint32_t cp = __builtin_amdgcn_perm(0u, 0x80u - p, 0u);
+        uint32_t q[4];
+        uint32_t r = 0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            q[t] = gather16(S, period_sel(r, pb, cp));
+            r += 4;
+            r = r >= p ? r - p : r;
+            r = r >= p ? r - p : r;
+        }
+        ring_put(w, rd, make_uint4(q[0], q[1], q[2], q[3]), len < 16 ? len : 16);
+        /* D - 1 for off = 1..15, a nibble each */
+        const uint32_t D = (uint32_t)((0x0EDCBA98FDBEFEFFull >> (4 * (off - 1))) & 15) + 1;
+        if (D == 16) {
+            for (uint32_t i = 16; i < len; i += 16) {
+                uint4 v;
+                __builtin_memcpy(&v, &w.ring[rd + i - 16], 16);
+                ring_put(w, rd + i, v, len - i < 16 ? len - i : 16);
+            }
+        } else {
+            for (uint32_t i = 16; i < len; i += 8) {
+                uint2 v;
+                __builtin_memcpy(&v, &w.ring[rd + i - D], 8);
+                ring_put(w, rd + i, make_uint4(v.x, v.y, 0, 0), len - i < 8 ? len - i : 8);
             }
         }
-        for (; i < len; i++) {
+    } else if (off >= 16 && rd + len <= kRing) {
+        for (uint32_t i = off; i < len; i += 16) {
+            uint4 v;
+            __builtin_memcpy(&v, &w.ring[rd + i - off], 16);
+            ring_put(w, rd + i, v, len - i < 16 ? len - i : 16);
+        }
+    } else {
+        for (uint32_t i = off; i < len; i++) {
             w.ring[(rd + i) & kRingMask] = w.ring[(rd + i - off) & kRingMask];
         }
     }
@@ -492,6 +581,19 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
     const uint64_t lit_at = D0 + drel;
     const uint64_t m_at = lit_at + lit_len;
     const bool wide_lit = lit_len > kLaneLit;
+    /* matches that read only history or their own literals go by their own
+     * lane; the history ones load their period now, so the load overlaps
+     * the literal phase */
+    const uint64_t y = m_at - offset;
+    const uint64_t need = match_len < offset ? match_len : offset;
+    const bool independent = y + need <= D0 || y >= lit_at;
+    const bool lane_match = full && !wide_lit && match_len <= kLaneMatch && independent;
+    const bool from_history =
+        lane_match && period_in_history(m_at, offset, match_len, hist_lo, cap);
+    Period hp;
+    if (from_history) {
+        hp = load_period(dst, y, (uint32_t)need);
+    }
     /* phase 1: literals, 16 bytes a step, the last step exact (a Snappy
      * literal has no match after it to absorb an overrun) */
     if (active && !wide_lit) {
@@ -515,21 +617,49 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
         }
     }
     wave_order();
-    /* phase 2: matches that read only history or their own literals, by
-     * their own lane */
-    const uint64_t y = m_at - offset;
-    const uint64_t need = match_len < offset ? match_len : offset;
-    const bool independent = y + need <= D0 || y >= lit_at;
-    const bool lane_match = full && !wide_lit && match_len <= kLaneMatch && independent;
+    /* phase 2: the lane matches */
     if (lane_match) {
-        lane_match_copy(w, dst, da, m_at, offset, match_len, hist_lo, cap);
+        lane_match_copy(w, dst, da, m_at, offset, match_len, hist_lo, cap, from_history, hp);
     }
     wave_order();
-    /* phase 3: the rest, in stream order, whole wave */
+    /* phase 3: the rest, in stream order, whole wave. The common member, a
+     * short match (<= 64 bytes) whose period is in the ring, is described by
+     * its lane up front in two words, so the loop pays two v_readlane and a
+     * byte a lane for it: fast_a = ring index of the match (12 bits), of
+     * its period (12 bits), length - 1 (6 bits) and the flag (bit 31);
+     * fast_b = min(offset, 64), as a float reciprocal's input. Byte i copies
+     * period byte i mod offset (i < 64; the float quotient is exact after
+     * one correction), which precedes the match, so the lanes do not depend
+     * on each other. */
+    const bool rest_fast = full && !lane_match && !wide_lit && match_len != 0 && match_len <= kWave &&
+                           y >= hist_lo;
+    const uint32_t fast_a = rest_fast ? (ring_index(da, m_at) | ring_index(da, y) << 12 |
+                                         (match_len - 1) << 24 | 1u << 31)
+                                      : 0u;
+    const uint32_t fast_b = offset < kWave ? offset : kWave;
+    const float lane_f = (float)lane;
     uint64_t rest = __ballot(active && (wide_lit || (full && !lane_match)));
     while (rest != 0) {
         const uint32_t j = __builtin_ctzll(rest);
         rest &= rest - 1;
+        const uint32_t ja = __builtin_amdgcn_readlane(fast_a, j);
+        if (ja >> 31) {
+            const uint32_t jo = __builtin_amdgcn_readlane(fast_b, j);
+            const uint32_t rm = ja & kRingMask;
+            const uint32_t rs = (ja >> 12) & kRingMask;
+            const uint32_t jm = ((ja >> 24) & 63) + 1;
+            uint32_t r = lane;
+            if (lane >= jo) {
+                const float inv = __builtin_amdgcn_rcpf((float)jo);
+                r = lane - __umul24(jo, (uint32_t)(lane_f * inv));
+                r = r >= jo ? r - jo : r;
+            }
+            if (lane < jm) {
+                w.ring[(rm + lane) & kRingMask] = w.ring[(rs + r) & kRingMask];
+            }
+            wave_order();
+            continue;
+        }
         const uint32_t jl = __builtin_amdgcn_readlane(lit_len, j);
         const uint32_t jlit = __builtin_amdgcn_readlane(lit, j);
         const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
@@ -547,28 +677,9 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
             if (!jdone) {
                 const uint64_t jm0 = jd0 + jl;
                 const uint64_t jfrom = jm0 - joff;
-                if (jm <= kWave && jfrom >= hist_lo) {
-                    /* the common case: a short match whose period is in the
-                     * ring, a byte a lane. Byte i copies period byte
-                     * i mod joff (i < 64; the float quotient is exact after
-                     * one correction), which precedes the match, so the
-                     * lanes do not depend on each other. */
-                    const uint32_t rs = ring_index(da, jfrom);
-                    const uint32_t rm = ring_index(da, jm0);
-                    uint32_t r = lane;
-                    if (lane >= joff) {
-                        const float inv = __builtin_amdgcn_rcpf((float)joff);
-                        r = lane - joff * (uint32_t)((float)lane * inv);
-                        r = r >= joff ? r - joff : r;
-                    }
-                    if (lane < jm) {
-                        w.ring[(rm + lane) & kRingMask] = w.ring[(rs + r) & kRingMask];
-                    }
-                } else {
-                    for (uint32_t i = lane; i < jm; i += kWave) {
-                        const uint64_t yy = jfrom + (i < joff ? i : i % joff);
-                        w.ring[ring_index(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
-                    }
+                for (uint32_t i = lane; i < jm; i += kWave) {
+                    const uint64_t yy = jfrom + (i < joff ? i : i % joff);
+                    w.ring[ring_index(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
                 }
             }
         }
