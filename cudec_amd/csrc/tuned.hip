@@ -138,44 +138,43 @@ struct Cand {
 /* The sequence a token at window position c would encode, assuming at most
  * one extension byte per length. Reads stay below c + 275, inside the stage. */
 __device__ __forceinline__ Cand parse_candidate(const uint8_t* cw, uint32_t c) {
+    /* branch-free: the token and the byte after it, then the three bytes
+     * at the offset (offset, match extension); bytes past the sequence are
+     * read but never used */
     Cand k;
     k.tok = cw[c];
-    uint32_t p = c + 1;
-    uint32_t lit = k.tok >> 4;
-    k.complex = false;
-    if (lit == 15) {
-        const uint32_t e = cw[p++];
-        lit += e;
-        k.complex = e == 255;
-    }
-    k.lit = p;
-    k.lit_len = lit;
-    p += lit;
+    const uint32_t e1 = cw[c + 1];
+    const uint32_t lh = k.tok >> 4;
+    const bool x1 = lh == 15;
+    k.lit_len = lh + (x1 ? e1 : 0);
+    k.lit = c + 1 + (x1 ? 1 : 0);
+    const uint32_t p = k.lit + k.lit_len;
     k.offset = (uint32_t)cw[p] | ((uint32_t)cw[p + 1] << 8);
-    p += 2;
-    uint32_t m = (k.tok & 15) + 4;
-    if ((k.tok & 15) == 15) {
-        const uint32_t e = cw[p++];
-        m += e;
-        k.complex = k.complex || e == 255;
-    }
-    k.match_len = m;
-    k.next = p;
+    const uint32_t e2 = cw[p + 2];
+    const uint32_t mh = k.tok & 15;
+    const bool x2 = mh == 15;
+    k.match_len = mh + 4 + (x2 ? e2 : 0);
+    k.next = p + 2 + (x2 ? 1 : 0);
+    k.complex = (x1 && e1 == 255) || (x2 && e2 == 255);
     return k;
 }
 
 /* ------------------------------------------------------------ wave scan */
 
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
-    /* Kogge-Stone over the wave with cross-lane moves; 6 steps. */
-#pragma unroll
-    for (uint32_t d = 1; d < kWave; d <<= 1) {
-        const uint32_t u = (uint32_t)__shfl_up((int)v, d, kWave);
-        if (lane >= d) {
-            v += u;
-        }
-    }
-    return v;
+    /* DPP: an inclusive scan inside each row of 16 lanes (row_shr 1, 2, 4,
+     * 8; a lane with no source adds the 0 it keeps), then lane 15 of each
+     * row into rows 1 and 3 (row_bcast:15) and lane 31 into rows 2 and 3
+     * (row_bcast:31). No LDS traffic. */
+    (void)lane;
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false); /* row_shr:1 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false); /* row_shr:2 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false); /* row_shr:4 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false); /* row_shr:8 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false); /* row_bcast:15 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false); /* row_bcast:31 */
+    return (uint32_t)x;
 }
 
 /* ----------------------------------------------------- ring <-> global */
