@@ -991,7 +991,7 @@ static_assert(kCandSplit * 64 + 5 + kSnappyWinLit + 16 <= stage_bytes(kCandSplit
 /* The element a tag at window position c would encode
  * (src/snappy_block.h:288-435): a literal (lit, lit_len; match_len 0) or a
  * copy-1/2/4 (match_len, offset; lit_len 0). One 8-byte read at the tag. */
-__device__ __forceinline__ Cand parse_candidate_snappy(const uint8_t* cw, uint32_t c) {
+__device__ __forceinline__ Cand snappy_element(const uint8_t* cw, uint32_t c) {
     Cand k;
     uint64_t h;
     __builtin_memcpy(&h, cw + c, 8);
@@ -1038,6 +1038,25 @@ __device__ __forceinline__ Cand parse_candidate_snappy(const uint8_t* cw, uint32
     return k;
 }
 
+/* The member a tag at window position c would start: one element, or a
+ * literal element fused with the copy element right after it (the pair an
+ * LZ4 sequence carries as literals + match), so a window's 64 members hold
+ * more of Snappy's short elements. A copy is fused only while its tag lies
+ * before the source end (lim = source bytes from the window start). */
+__device__ __forceinline__ Cand parse_candidate_snappy(const uint8_t* cw, uint32_t c,
+                                                      uint32_t lim) {
+    Cand k = snappy_element(cw, c);
+    if (k.match_len == 0 && !k.complex && k.next < lim) {
+        const Cand m = snappy_element(cw, k.next);
+        if (m.match_len != 0) {
+            k.match_len = m.match_len;
+            k.offset = m.offset;
+            k.next = m.next;
+        }
+    }
+    return k;
+}
+
 /* As parse_window, for Snappy elements after the preamble: every member's
  * rungs run in parallel (every reject is CORRUPT_INPUT, so the first
  * failing member in stream order decides), and the member whose position
@@ -1057,11 +1076,14 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
     const uint64_t base = sp;
     out.delta = stage_into<kC>(stage, src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
+    /* source bytes from the window start, clamped: a fused copy must start
+     * before the source end */
+    const uint32_t lim = src_size - base >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(src_size - base);
 
     uint32_t link[kC];
 #pragma unroll
     for (uint32_t k = 0; k < kC; k++) {
-        const Cand c = parse_candidate_snappy(cw, k * kWave + lane);
+        const Cand c = parse_candidate_snappy(cw, k * kWave + lane, lim);
         link[k] = c.complex ? kStop : c.next;
     }
     /* a window that opens on a literal it cannot hold: the scalar ladder
@@ -1102,7 +1124,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
     bool slow = count == 0;
     if (!slow) {
         const bool is_member = lane < count;
-        const Cand c = parse_candidate_snappy(cw, is_member ? member : 0);
+        const Cand c = parse_candidate_snappy(cw, is_member ? member : 0, lim);
         const uint64_t s = base + member;
         const bool at_end = is_member && s >= src_size;
         const uint32_t out_len = is_member && !at_end ? c.lit_len + c.match_len : 0;
@@ -1115,7 +1137,9 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
                 ev = s == src_size && d0 == declared ? kOk : kCorrupt;
             } else if (d0 > declared) {
                 ev = kCorrupt;
-            } else if (c.match_len == 0) {
+            } else if (c.lit_len != 0 || c.match_len == 0) {
+                /* a literal element (src/snappy_block.h:345-370), then, in a
+                 * fused member, the copy element after it at (s2, d2) */
                 const uint64_t header = c.lit - member;
                 if (header > src_size - s) {
                     ev = kCorrupt;
@@ -1123,8 +1147,20 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
                     ev = kCorrupt;
                 } else if (c.lit_len > declared - d0) {
                     ev = kCorrupt;
+                } else if (c.match_len != 0) {
+                    const uint64_t s2 = s + header + c.lit_len;
+                    const uint64_t d2 = d0 + c.lit_len;
+                    const uint64_t header2 = c.next - (c.lit + c.lit_len);
+                    if (header2 > src_size - s2) {
+                        ev = kCorrupt;
+                    } else if (c.offset == 0 || (uint64_t)c.offset > d2) {
+                        ev = kCorrupt;
+                    } else if (c.match_len > declared - d2) {
+                        ev = kCorrupt;
+                    }
                 }
             } else {
+                /* a copy element (src/snappy_block.h:371-435) */
                 const uint64_t header = c.lit - member;
                 if (header > src_size - s) {
                     ev = kCorrupt;
