@@ -439,7 +439,7 @@ def main():
             db.dst_caps.data_ptr(), db.n, db.d_results.data_ptr(), stream.cuda_stream)
         if st != 0:
             raise SystemExit(f"[bench] autotune failed: {st}")
-        tuning = {"knob": "LZ4 chunks with cost (compressed + capacity) >= threshold decode "
+        tuning = {"knob": "LZ4 chunks with cost (2 x capacity - compressed) >= threshold decode "
                           "on two pipelined waves, cheaper ones on one",
                   "best_threshold_bytes": best,
                   "sweep_ms": {("all" if c == 0 else "none" if c >= 2**63 else str(c)):

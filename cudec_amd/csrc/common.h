@@ -91,8 +91,8 @@ hipError_t launch_plain(int format, const void* const* srcs,
                         cudec_chunk_result* results, hipStream_t stream);
 
 /* The current device's LZ4 pipelining threshold (tuned.hip): chunks whose
- * cost (compressed size + capacity) is at least this many bytes decode on
- * two waves; 0 means every chunk, UINT64_MAX none. */
+ * cost (capacity + (capacity - compressed size)) is at least this many
+ * bytes decode on two waves; 0 means every chunk, UINT64_MAX none. */
 hipError_t set_lz4_pipe_min_cost(uint64_t min_cost);
 hipError_t get_lz4_pipe_min_cost(uint64_t* min_cost);
 

@@ -1,8 +1,9 @@
 /* Autotuner of the LZ4 batch launch: which chunks decode on two waves
  * (parse and execute pipelined) and which on one.
  *
- * Chunks are ordered by cost (compressed size + capacity) inside a launch;
- * a chunk at or above the threshold cost runs on a two-wave block, a
+ * Chunks are ordered by cost (capacity + (capacity - compressed size))
+ * inside a launch; a chunk at or above the threshold cost runs on a
+ * two-wave block, a
  * cheaper one on a single wave. Two waves shorten one chunk's critical path
  * (large chunks bound a launch); one wave keeps twice as many chunks in
  * flight (batches of small chunks are throughput-bound). The tuner decodes

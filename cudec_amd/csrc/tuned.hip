@@ -1,8 +1,8 @@
 /* The tuned launch form of the LZ4 and Snappy batch decoders for gfx950.
  *
  * Work distribution: four small kernels bin the chunks by decode cost
- * (compressed size + capacity; four bins per octave) into a device queue,
- * costliest first. A persistent grid of two-wave blocks then pulls chunks
+ * (capacity + (capacity - compressed size), the output plus what matches
+ * add to it; four bins per octave) into a device queue, costliest first. A persistent grid of two-wave blocks then pulls chunks
  * from it. A chunk at or above the tuned cost threshold runs on both waves
  * of its block, as a software pipeline. Once the queue reaches cheaper
  * chunks, a block splits into two independent one-wave decoders.
@@ -66,7 +66,7 @@ constexpr uint32_t kStop = 0xFFFFu;       /* link of an unresolvable candidate *
 constexpr uint32_t kLaneLit = 64;         /* literal runs copied by their lane */
 constexpr uint32_t kLaneMatch = 64;       /* matches copied by their lane (<= 64) */
 constexpr int kBins = 256;
-/* Default: chunks costing at least 512 KiB (compressed size + capacity) run
+/* Default: chunks costing at least 512 KiB (capacity + capacity - compressed) run
  * on two waves. */
 constexpr uint64_t kDefaultPipeMinCost = 512u << 10;
 
@@ -1385,10 +1385,12 @@ __global__ void queue_reset(Queue* q) {
  * literals and far matches). Four bins per octave; larger cost, smaller
  * bin, decoded first. */
 __device__ __forceinline__ uint32_t cost_bin(uint64_t src_size, uint64_t cap) {
-    const uint64_t lim = 1ull << 62;
+    const uint64_t lim = 1ull << 61;
     const uint64_t a = src_size < lim ? src_size : lim;
     const uint64_t b = cap < lim ? cap : lim;
-    const uint64_t cost = a + b;
+    /* the output, plus what matches add to it (capacity - compressed):
+     * match bytes cost the decode its time, literal bytes are a copy */
+    const uint64_t cost = b + (b > a ? b - a : 0);
     if (cost < 4) {
         return kBins - 1 - (uint32_t)cost;
     }

@@ -40,7 +40,8 @@ size_t cudec_mi355x_max_batch_chunks(void);
 
 /* No counterpart in the reference: launch tuning of the LZ4 batch entry
  * (cudec_lz4_decompress_batch) on the current device. Inside one launch the
- * chunks run costliest first (cost = compressed size + capacity); a chunk
+ * chunks run costliest first (cost = capacity + (capacity - compressed
+ * size): the output plus what matches add to it); a chunk
  * whose cost is at least `min_cost` bytes decodes on two waves (one
  * parses the next window while the other executes the current one), a
  * cheaper one on a single wave. 0 puts every chunk on two waves, SIZE_MAX
