@@ -1437,8 +1437,12 @@ constexpr uint32_t kNoFin = 0xFFFFFFFFu;
  * both waves read the chunk index and the final step from LDS after a
  * barrier, and a barrier opens every chunk so that no control word is
  * rewritten while the other wave may still read it. */
-template <int kFormat>
-__global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu(4)))
+/* kWaves: waves per SIMD the register budget is cut for. 4 (128 VGPRs) for
+ * launches that pipeline chunks; 5 (96 VGPRs, some spills on the two-wave
+ * path) for launches where every chunk runs on one wave, which are
+ * throughput-bound and gain from a fifth wave. */
+template <int kFormat, int kWaves>
+__global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu(kWaves)))
     piped_batch(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
                 const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
                 const uint32_t* order, int32_t pipe_bin) {
@@ -1596,7 +1600,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
  * stream of launches does not return memory to the driver between them. */
 struct DeviceState {
     int cus = 0;
-    int pipe_blocks_per_cu = 0;  /* piped_batch (two-wave blocks) */
+    int pipe_blocks_per_cu = 0;  /* piped_batch<_, 4> (two-wave blocks) */
+    int split_blocks_per_cu = 0; /* piped_batch<_, 5>, launches with no two-wave chunk */
     uint64_t lz4_pipe_min_cost = kDefaultPipeMinCost; /* two waves at or above this cost */
     hipMemPool_t pool = nullptr;
     bool ready = false;
@@ -1640,12 +1645,18 @@ hipError_t device_state(DeviceState** out) {
             return e;
         }
         int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4>, kPipeBlock,
-                                                         0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 4>,
+                                                         kPipeBlock, 0);
         if (e != hipSuccess) {
             return e;
         }
         d.pipe_blocks_per_cu = per > 0 ? per : 1;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 5>,
+                                                         kPipeBlock, 0);
+        if (e != hipSuccess) {
+            return e;
+        }
+        d.split_blocks_per_cu = per > 0 ? per : 1;
         hipMemPoolProps props = {};
         props.allocType = hipMemAllocationTypePinned;
         props.location.type = hipMemLocationTypeDevice;
@@ -1733,17 +1744,31 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q, order);
     const int32_t pipe_bin = pipe_bin_for(min_cost);
     /* two-wave blocks; a block splits into two one-wave decoders when it
-     * meets a chunk below the threshold */
-    size_t grid = (size_t)dev->cus * (size_t)dev->pipe_blocks_per_cu;
+     * meets a chunk below the threshold. With no two-wave chunk at all the
+     * five-wave build runs (every block splits at once). */
+    const bool all_split = pipe_bin < 0;
+    size_t grid = (size_t)dev->cus *
+                  (size_t)(all_split ? dev->split_blocks_per_cu : dev->pipe_blocks_per_cu);
     if (grid > n) {
         grid = n;
     }
+    const dim3 gd((unsigned)grid), bd(kPipeBlock);
     if (format == kFormatSnappy) {
-        hipLaunchKernelGGL(piped_batch<kFormatSnappy>, dim3((unsigned)grid), dim3(kPipeBlock), 0,
-                           stream, srcs, src_sizes, dsts, caps, n, results, q, order, pipe_bin);
+        if (all_split) {
+            hipLaunchKernelGGL((piped_batch<kFormatSnappy, 5>), gd, bd, 0, stream, srcs, src_sizes,
+                               dsts, caps, n, results, q, order, pipe_bin);
+        } else {
+            hipLaunchKernelGGL((piped_batch<kFormatSnappy, 4>), gd, bd, 0, stream, srcs, src_sizes,
+                               dsts, caps, n, results, q, order, pipe_bin);
+        }
     } else {
-        hipLaunchKernelGGL(piped_batch<kFormatLz4>, dim3((unsigned)grid), dim3(kPipeBlock), 0,
-                           stream, srcs, src_sizes, dsts, caps, n, results, q, order, pipe_bin);
+        if (all_split) {
+            hipLaunchKernelGGL((piped_batch<kFormatLz4, 5>), gd, bd, 0, stream, srcs, src_sizes,
+                               dsts, caps, n, results, q, order, pipe_bin);
+        } else {
+            hipLaunchKernelGGL((piped_batch<kFormatLz4, 4>), gd, bd, 0, stream, srcs, src_sizes,
+                               dsts, caps, n, results, q, order, pipe_bin);
+        }
     }
     e = hipGetLastError();
     const hipError_t f = hipFreeAsync(scratch, stream);
