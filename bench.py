@@ -62,6 +62,9 @@ def parse_args():
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default LZ4 pipelining threshold instead of tuning it "
                         "on the batch before timing")
+    p.add_argument("--threshold", type=int, default=None,
+                   help="set the LZ4 pipelining threshold (bytes of cost; -1: none) instead of "
+                        "tuning it, so a profiled run repeats one launch shape")
     p.add_argument("--serve-rates", default="2000,8000,32000",
                    help="request rates (per second) of the serving leg; empty to skip it")
     p.add_argument("--serve-seconds", type=float, default=0.5,
@@ -431,7 +434,13 @@ def main():
                       and args.seed == 0x5EED and forms[0] == "tuned")
     measured = {}
     tuning = None
-    if args.format == "lz4" and "tuned" in forms and not args.no_autotune:
+    if args.threshold is not None:
+        import cudec_amd
+        t = (1 << 64) - 1 if args.threshold < 0 else args.threshold
+        if cudec_amd.set_lz4_pipeline_threshold(t) != 0:
+            raise SystemExit("[bench] could not set the pipelining threshold")
+        log(f"pipelining threshold set to {t}")
+    elif args.format == "lz4" and "tuned" in forms and not args.no_autotune:
         import cudec_amd
         t0 = time.perf_counter()
         st, best, sweep = cudec_amd.autotune_lz4(
