@@ -60,7 +60,6 @@ constexpr uint32_t kCandPipe = 16;
 constexpr uint32_t stage_bytes(uint32_t kc) { return kc * 64u + 512u; }
 constexpr uint32_t kStageBuf = stage_bytes(kCandPipe) + 16;
 constexpr uint32_t kRing = 4096;          /* per-wave output ring */
-constexpr uint32_t kRingMask = kRing - 1;
 constexpr uint32_t kBatchMax = 2048;      /* output bytes of one batch */
 constexpr uint32_t kStop = 0xFFFFu;       /* link of an unresolvable candidate */
 constexpr uint32_t kLaneLit = 64;         /* literal runs copied by their lane */
@@ -71,15 +70,27 @@ constexpr int kBins = 256;
 constexpr uint64_t kDefaultPipeMinCost = 512u << 10;
 
 static_assert(kBatchMax + 32 <= kRing, "ring must hold a batch and a margin");
+static_assert((kRing & (kRing - 1)) == 0, "rings are powers of two");
 static_assert(kLaneMatch <= 64, "a lane match copies at most four 16-byte blocks");
 static_assert(kCandSplit * 64 + 2 + 269 + 3 <= stage_bytes(kCandSplit) &&
                   kCandPipe * 64 + 2 + 269 + 3 <= stage_bytes(kCandPipe),
               "a window's last candidate fits the stage");
 
-struct alignas(16) WaveLds {
-    uint8_t ring[kRing];
+/* A wave's output ring (kR bytes, a power of two) and a stage buffer. The
+ * one-wave decoders keep a 4 KiB ring each and the link table of their
+ * window (walk_doubling); the two-wave form's copier an 8 KiB ring in the
+ * same LDS (the two never run at once in a block). */
+template <uint32_t kR>
+struct alignas(16) RingLds {
+    static constexpr uint32_t kSize = kR;
+    uint8_t ring[kR];
     uint8_t stage[kStageBuf];
 };
+struct alignas(16) WaveLds : RingLds<kRing> {
+    uint16_t jump[kCandSplit * kWave];
+};
+constexpr uint32_t kPipeRing = 8192;
+using PipeRing = RingLds<kPipeRing>;
 
 /* Scratch of one tuned launch (stream-ordered allocation). */
 struct Queue {
@@ -90,8 +101,9 @@ struct Queue {
     uint32_t pad[3];
 };
 
+template <uint32_t kR = kRing>
 __device__ __forceinline__ uint32_t ring_index(uintptr_t dst_addr, uint64_t pos) {
-    return (uint32_t)(dst_addr + pos) & kRingMask;
+    return (uint32_t)(dst_addr + pos) & (kR - 1);
 }
 
 /* ------------------------------------------------------------ staging */
@@ -182,7 +194,8 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane
 /* Stores ring bytes of positions [from, to) to dst. Full aligned 16-byte
  * blocks go out as one vector store per lane; the partial blocks at either
  * end (only at the chunk's first byte or its last) as single bytes. */
-__device__ __forceinline__ void flush_ring(const WaveLds& w, gu8* dst,
+template <class W>
+__device__ __forceinline__ void flush_ring(const W& w, gu8* dst,
                                            uint64_t from, uint64_t to,
                                            uint32_t lane) {
     if (to <= from) {
@@ -204,15 +217,15 @@ __device__ __forceinline__ void flush_ring(const WaveLds& w, gu8* dst,
     const uint32_t tail = (uint32_t)(a1 - b1);
     if (lane < head) {
         const uintptr_t g = a0 + lane;
-        *(gu8*)g = w.ring[g & kRingMask];
+        *(gu8*)g = w.ring[g & (W::kSize - 1)];
     } else if (lane >= 16 && lane < 16 + tail) {
         const uintptr_t g = b1 + (lane - 16);
-        *(gu8*)g = w.ring[g & kRingMask];
+        *(gu8*)g = w.ring[g & (W::kSize - 1)];
     }
     const uint32_t blocks = (uint32_t)((b1 - b0) / 16);
     for (uint32_t b = lane; b < blocks; b += kWave) {
         const uintptr_t g = b0 + 16u * b;
-        const uint4 v = *reinterpret_cast<const uint4*>(&w.ring[g & kRingMask]);
+        const uint4 v = *reinterpret_cast<const uint4*>(&w.ring[g & (W::kSize - 1)]);
         gstore16((gu8*)g, v);
     }
     wave_order();
@@ -223,10 +236,11 @@ __device__ __forceinline__ void flush_ring(const WaveLds& w, gu8* dst,
  * global read takes the aligned 4-byte word holding the byte: a different
  * load width keeps the compiler from merging the two branches into one
  * flat load, and an aligned word never leaves the page of the byte. */
-__device__ __forceinline__ uint8_t history_byte(const WaveLds& w, gcu8* dst,
+template <class W>
+__device__ __forceinline__ uint8_t history_byte(const W& w, gcu8* dst,
                                                 uint64_t y, uint64_t ring_lo) {
     if (y >= ring_lo) {
-        return w.ring[ring_index((uintptr_t)dst, y)];
+        return w.ring[ring_index<W::kSize>((uintptr_t)dst, y)];
     }
     const uintptr_t a = (uintptr_t)(dst + y);
     const uint32_t word = *(gcu32*)(a & ~(uintptr_t)3);
@@ -241,11 +255,12 @@ __device__ __forceinline__ uint32_t block_byte(const uint4& v, uint32_t i) {
 
 /* Exactly n (0..16) bytes of v to ring position r, wrapping at the ring's
  * end: 8/4/2/1-byte stores, so no byte past n is touched. */
-__device__ __forceinline__ void ring_put(WaveLds& w, uint32_t r, const uint4& v, uint32_t n) {
+template <class W>
+__device__ __forceinline__ void ring_put(W& w, uint32_t r, const uint4& v, uint32_t n) {
     if (n == 0) {
         return;
     }
-    if (r + n <= kRing) {
+    if (r + n <= W::kSize) {
         uint8_t* p = &w.ring[r];
         if (n == 16) {
             __builtin_memcpy(p, &v, 16);
@@ -276,7 +291,7 @@ __device__ __forceinline__ void ring_put(WaveLds& w, uint32_t r, const uint4& v,
         return;
     }
     for (uint32_t i = 0; i < n; i++) {
-        w.ring[(r + i) & kRingMask] = (uint8_t)block_byte(v, i);
+        w.ring[(r + i) & (W::kSize - 1)] = (uint8_t)block_byte(v, i);
     }
 }
 
@@ -342,16 +357,17 @@ __device__ __forceinline__ Period load_period(gcu8* dst, uint64_t y, uint32_t ne
  *     the rest is copied from D = off * floor(16 / off) bytes back, 8 or 16
  *     bytes a step; for off >= 16, 16 bytes a step from off back. Where the
  *     match wraps the ring, bytewise. */
-__device__ __forceinline__ void lane_match_copy(WaveLds& w, gcu8* dst, uintptr_t da,
+template <class W>
+__device__ __forceinline__ void lane_match_copy(W& w, gcu8* dst, uintptr_t da,
                                                 uint64_t m, uint32_t off, uint32_t len,
                                                 uint64_t hist_lo, uint64_t cap, bool from_history,
                                                 const Period& hp) {
     const uint64_t y = m - off;
     const uint32_t need = len < off ? len : off;
-    const uint32_t rd = ring_index(da, m);
-    const uint32_t ry = ring_index(da, y);
+    const uint32_t rd = ring_index<W::kSize>(da, m);
+    const uint32_t ry = ring_index<W::kSize>(da, y);
     /* step 1 */
-    if (from_history || (y >= hist_lo && ry + need <= kRing)) {
+    if (from_history || (y >= hist_lo && ry + need <= W::kSize)) {
         uint4 h0, h1, h2, h3;
         if (from_history) {
             h0 = hp.h0;
@@ -375,24 +391,24 @@ __device__ __forceinline__ void lane_match_copy(WaveLds& w, gcu8* dst, uintptr_t
         }
         ring_put(w, rd, h0, need < 16 ? need : 16);
         if (need > 16) {
-            ring_put(w, (rd + 16) & kRingMask, h1, need - 16 < 16 ? need - 16 : 16);
+            ring_put(w, (rd + 16) & (W::kSize - 1), h1, need - 16 < 16 ? need - 16 : 16);
         }
         if (need > 32) {
-            ring_put(w, (rd + 32) & kRingMask, h2, need - 32 < 16 ? need - 32 : 16);
+            ring_put(w, (rd + 32) & (W::kSize - 1), h2, need - 32 < 16 ? need - 32 : 16);
         }
         if (need > 48) {
-            ring_put(w, (rd + 48) & kRingMask, h3, need - 48);
+            ring_put(w, (rd + 48) & (W::kSize - 1), h3, need - 48);
         }
     } else {
         for (uint32_t i = 0; i < need; i++) {
-            w.ring[(rd + i) & kRingMask] = history_byte(w, dst, y + i, hist_lo);
+            w.ring[(rd + i) & (W::kSize - 1)] = history_byte(w, dst, y + i, hist_lo);
         }
     }
     if (need == len) {
         return;
     }
     /* step 2: off < len */
-    if (off < 16 && rd + (len > 16 ? len : 16) <= kRing) {
+    if (off < 16 && rd + (len > 16 ? len : 16) <= W::kSize) {
         uint4 S;
         __builtin_memcpy(&S, &w.ring[rd], 16);
         uint32_t p = off;
@@ -428,7 +444,7 @@ __device__ __forceinline__ void lane_match_copy(WaveLds& w, gcu8* dst, uintptr_t
                 ring_put(w, rd + i, make_uint4(v.x, v.y, 0, 0), len - i < 8 ? len - i : 8);
             }
         }
-    } else if (off >= 16 && rd + len <= kRing) {
+    } else if (off >= 16 && rd + len <= W::kSize) {
         for (uint32_t i = off; i < len; i += 16) {
             uint4 v;
             __builtin_memcpy(&v, &w.ring[rd + i - off], 16);
@@ -436,7 +452,7 @@ __device__ __forceinline__ void lane_match_copy(WaveLds& w, gcu8* dst, uintptr_t
         }
     } else {
         for (uint32_t i = off; i < len; i++) {
-            w.ring[(rd + i) & kRingMask] = w.ring[(rd + i - off) & kRingMask];
+            w.ring[(rd + i) & (W::kSize - 1)] = w.ring[(rd + i - off) & (W::kSize - 1)];
         }
     }
 }
@@ -484,7 +500,8 @@ static_assert(kStagedPeriod + 16 <= kRing, "a staged period fits the ring");
  *    byte-per-lane gather pays one per 64 bytes and whole-period rounds one
  *    per period;
  *  - otherwise: rounds of off (>= kStagedPeriod) bytes. */
-__device__ __forceinline__ void wave_match_global(WaveLds& w, gu8* dst, uint64_t m,
+template <class W>
+__device__ __forceinline__ void wave_match_global(W& w, gu8* dst, uint64_t m,
                                                   uint64_t off, uint64_t len, uint64_t cap,
                                                   uint32_t lane) {
     if (off >= len) {
@@ -560,13 +577,71 @@ struct Group {
     uint32_t end_rel;
 };
 
+/* One lane's literal of lit_len (<= kLaneLit) bytes, staged at cw + lit,
+ * to the ring at output position lit_at: 16 bytes a step, the last step
+ * exact (a Snappy literal has no match after it to absorb an overrun). */
+template <class W>
+__device__ __forceinline__ void put_literal(W& w, const uint8_t* cw, uint32_t lit,
+                                            uint32_t lit_len, uintptr_t da, uint64_t lit_at) {
+    uint32_t i0 = 0;
+    for (; i0 + 16 <= lit_len; i0 += 16) {
+        uint4 v;
+        __builtin_memcpy(&v, cw + lit + i0, 16);
+        const uint32_t r = ring_index<W::kSize>(da, lit_at + i0);
+        if (r <= W::kSize - 16) {
+            __builtin_memcpy(&w.ring[r], &v, 16);
+        } else {
+            for (uint32_t b = 0; b < 16; b++) {
+                w.ring[(r + b) & (W::kSize - 1)] = (uint8_t)block_byte(v, b);
+            }
+        }
+    }
+    if (i0 < lit_len) {
+        uint4 v;
+        __builtin_memcpy(&v, cw + lit + i0, 16);
+        ring_put(w, ring_index<W::kSize>(da, lit_at + i0), v, lit_len - i0);
+    }
+}
+
+/* Every literal of group g (window staged at cw, group output starting at
+ * D0) to its place in the ring: short ones by their lane, longer ones by
+ * the whole wave. The two-wave form runs this on the parsing wave for the
+ * window after the one being executed, so that the copying wave finds the
+ * literals in place. */
+template <class W>
+__device__ __forceinline__ void place_literals(W& w, const uint8_t* cw, const Group& g,
+                                               uintptr_t da, uint64_t D0, uint32_t lane) {
+    const bool active = lane < g.take || (g.terminal && lane == g.take);
+    const bool wide_lit = g.lit_len > kLaneLit;
+    if (active && !wide_lit) {
+        put_literal(w, cw, g.lit, g.lit_len, da, D0 + g.drel);
+    }
+    uint64_t wide = __ballot(active && wide_lit);
+    while (wide != 0) {
+        const uint32_t j = __builtin_ctzll(wide);
+        wide &= wide - 1;
+        const uint32_t jl = __builtin_amdgcn_readlane(g.lit_len, j);
+        const uint32_t jlit = __builtin_amdgcn_readlane(g.lit, j);
+        const uint64_t jd0 = D0 + __builtin_amdgcn_readlane(g.drel, j);
+        for (uint32_t i = lane; i < jl; i += kWave) {
+            w.ring[ring_index<W::kSize>(da, jd0 + i)] = cw[jlit + i];
+        }
+    }
+    wave_order();
+}
+
 /* Executes group g of the window staged at cw into the ring at output
  * positions [dp, dp + g.end_rel), flushes whole 16-byte blocks (all of them
- * when the chunk is done) and advances dp. */
-__device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, const Group& g,
+ * when the chunk is done) and advances dp. placed: every literal of the
+ * group is in the ring already (place_literals). reserve: ring bytes past
+ * the group's end that another wave may be writing meanwhile, so history
+ * reads stay below them. */
+template <class W>
+__device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Group& g,
                                               gu8* dst, uint64_t cap, uint64_t& dp,
                                               uint64_t& flushed, uint64_t ring_lo, bool done,
-                                              uint32_t lane) {
+                                              uint32_t lane, bool placed = false,
+                                              uint32_t reserve = 0) {
     const uintptr_t da = (uintptr_t)dst;
     const uint32_t take = g.take;
     const bool full = lane < take;
@@ -575,11 +650,12 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
                    offset = g.offset, drel = g.drel;
     const uint64_t D0 = dp;
     const uint64_t D1 = dp + g.end_rel;
-    const uint64_t ring_floor = D1 + 16 > kRing ? D1 + 16 - kRing : 0;
+    const uint64_t ring_floor = D1 + 16 + reserve > W::kSize ? D1 + 16 + reserve - W::kSize : 0;
     const uint64_t hist_lo = ring_lo > ring_floor ? ring_lo : ring_floor;
     const uint64_t lit_at = D0 + drel;
     const uint64_t m_at = lit_at + lit_len;
-    const bool wide_lit = lit_len > kLaneLit;
+    /* a literal the whole wave copies in stream order (none once placed) */
+    const bool wide_lit = !placed && lit_len > kLaneLit;
     /* matches that read only history or their own literals go by their own
      * lane; the history ones load their period now, so the load overlaps
      * the literal phase */
@@ -593,27 +669,9 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
     if (from_history) {
         hp = load_period(dst, y, (uint32_t)need);
     }
-    /* phase 1: literals, 16 bytes a step, the last step exact (a Snappy
-     * literal has no match after it to absorb an overrun) */
-    if (active && !wide_lit) {
-        uint32_t i0 = 0;
-        for (; i0 + 16 <= lit_len; i0 += 16) {
-            uint4 v;
-            __builtin_memcpy(&v, cw + lit + i0, 16);
-            const uint32_t r = ring_index(da, lit_at + i0);
-            if (r <= kRing - 16) {
-                __builtin_memcpy(&w.ring[r], &v, 16);
-            } else {
-                for (uint32_t b = 0; b < 16; b++) {
-                    w.ring[(r + b) & kRingMask] = (uint8_t)block_byte(v, b);
-                }
-            }
-        }
-        if (i0 < lit_len) {
-            uint4 v;
-            __builtin_memcpy(&v, cw + lit + i0, 16);
-            ring_put(w, ring_index(da, lit_at + i0), v, lit_len - i0);
-        }
+    /* phase 1: the lanes' literals */
+    if (!placed && active && !wide_lit) {
+        put_literal(w, cw, lit, lit_len, da, lit_at);
     }
     wave_order();
     /* phase 2: the lane matches */
@@ -624,29 +682,29 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
     /* phase 3: the rest, in stream order, whole wave. The common member, a
      * short match (<= 64 bytes) whose period is in the ring, is described by
      * its lane up front in two words, so the loop pays two v_readlane and a
-     * byte a lane for it: fast_a = ring index of the match (12 bits), of
-     * its period (12 bits), length - 1 (6 bits) and the flag (bit 31);
-     * fast_b = min(offset, 64), as a float reciprocal's input. Byte i copies
-     * period byte i mod offset (i < 64; the float quotient is exact after
-     * one correction), which precedes the match, so the lanes do not depend
-     * on each other. */
+     * byte a lane for it: fast_a = ring index of the match (13 bits), of
+     * its period (13 bits), length - 1 (6 bits); fast_b = min(offset, 64)
+     * (7 bits) and the flag (bit 31). Byte i copies period byte i mod
+     * offset (i < 64; the float quotient is exact after one correction),
+     * which precedes the match, so the lanes do not depend on each other. */
     const bool rest_fast = full && !lane_match && !wide_lit && match_len != 0 && match_len <= kWave &&
                            y >= hist_lo;
-    const uint32_t fast_a = rest_fast ? (ring_index(da, m_at) | ring_index(da, y) << 12 |
-                                         (match_len - 1) << 24 | 1u << 31)
-                                      : 0u;
-    const uint32_t fast_b = offset < kWave ? offset : kWave;
+    static_assert(W::kSize <= 8192, "ring indices are packed in 13 bits");
+    const uint32_t fast_a = ring_index<W::kSize>(da, m_at) | ring_index<W::kSize>(da, y) << 13 |
+                            (match_len - 1) << 26;
+    const uint32_t fast_b = (offset < kWave ? offset : kWave) | (rest_fast ? 1u << 31 : 0u);
     const float lane_f = (float)lane;
     uint64_t rest = __ballot(active && (wide_lit || (full && !lane_match)));
     while (rest != 0) {
         const uint32_t j = __builtin_ctzll(rest);
         rest &= rest - 1;
-        const uint32_t ja = __builtin_amdgcn_readlane(fast_a, j);
-        if (ja >> 31) {
-            const uint32_t jo = __builtin_amdgcn_readlane(fast_b, j);
-            const uint32_t rm = ja & kRingMask;
-            const uint32_t rs = (ja >> 12) & kRingMask;
-            const uint32_t jm = ((ja >> 24) & 63) + 1;
+        const uint32_t jb = __builtin_amdgcn_readlane(fast_b, j);
+        if (jb >> 31) {
+            const uint32_t ja = __builtin_amdgcn_readlane(fast_a, j);
+            const uint32_t jo = jb & 127;
+            const uint32_t rm = ja & 8191;
+            const uint32_t rs = (ja >> 13) & 8191;
+            const uint32_t jm = (ja >> 26) + 1;
             uint32_t r = lane;
             if (lane >= jo) {
                 const float inv = __builtin_amdgcn_rcpf((float)jo);
@@ -654,7 +712,7 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
                 r = r >= jo ? r - jo : r;
             }
             if (lane < jm) {
-                w.ring[(rm + lane) & kRingMask] = w.ring[(rs + r) & kRingMask];
+                w.ring[(rm + lane) & (W::kSize - 1)] = w.ring[(rs + r) & (W::kSize - 1)];
             }
             wave_order();
             continue;
@@ -663,9 +721,9 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
         const uint32_t jlit = __builtin_amdgcn_readlane(lit, j);
         const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
         const uint64_t jd0 = D0 + jd;
-        if (jl > kLaneLit) {
+        if (!placed && jl > kLaneLit) {
             for (uint32_t i = lane; i < jl; i += kWave) {
-                w.ring[ring_index(da, jd0 + i)] = cw[jlit + i];
+                w.ring[ring_index<W::kSize>(da, jd0 + i)] = cw[jlit + i];
             }
         }
         wave_order();
@@ -678,7 +736,7 @@ __device__ __forceinline__ void execute_group(WaveLds& w, const uint8_t* cw, con
                 const uint64_t jfrom = jm0 - joff;
                 for (uint32_t i = lane; i < jm; i += kWave) {
                     const uint64_t yy = jfrom + (i < joff ? i : i % joff);
-                    w.ring[ring_index(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
+                    w.ring[ring_index<W::kSize>(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
                 }
             }
         }
@@ -760,6 +818,75 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
     return uniform32(pos);
 }
 
+/* The chain from `start`, as walk_chain decides it (members while in the
+ * window and under 64; a position with a stop link is no member), by
+ * pointer doubling over the window's links in LDS (jump, kC * 64 entries)
+ * instead of one dependent step per member. Round r (r = 0..5) has the
+ * table map a position to the one 2^r links on: lanes [2^r, 2^(r+1)) take
+ * their member from lane (lane - 2^r) with one lookup, then every entry
+ * looks itself up to double the reach. A link at or past the window end
+ * (kStop included) maps to itself. The rounds stop once the last member
+ * found is past the window. `link` is clobbered. */
+template <uint32_t kC>
+__device__ __forceinline__ uint32_t walk_doubling(uint32_t (&link)[kC], uint32_t start,
+                                                  uint16_t* jump, uint32_t& count_out,
+                                                  uint32_t& member, uint32_t lane) {
+    constexpr uint32_t kW = kC * kWave;
+    static_assert(kStop >= kW && kStop <= 0xFFFFu, "a stop link is past every window");
+#pragma unroll
+    for (uint32_t k = 0; k < kC; k++) {
+        jump[k * kWave + lane] = (uint16_t)link[k];
+    }
+    wave_order();
+    uint32_t m = start; /* lane i: member i's position once found */
+    uint32_t h = 1;
+    for (;;) {
+        const uint32_t from = (uint32_t)__shfl((int)m, (int)((lane - h) & (kWave - 1)));
+        if (lane >= h && lane < 2 * h) {
+            m = from < kW ? jump[from] : from;
+        }
+        const uint32_t last = uniform32(__builtin_amdgcn_readlane(m, 2 * h - 1));
+        if (last >= kW || h == kWave / 2) {
+            break;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kC; k++) {
+            link[k] = link[k] < kW ? jump[link[k]] : link[k];
+        }
+        wave_order();
+#pragma unroll
+        for (uint32_t k = 0; k < kC; k++) {
+            jump[k * kWave + lane] = (uint16_t)link[k];
+        }
+        wave_order();
+        h *= 2;
+    }
+    const uint64_t found = 2 * h >= kWave ? ~0ull : (1ull << (2 * h)) - 1;
+    const uint64_t past = __ballot(m >= kW) & found;
+    uint32_t count, pos;
+    if (past != 0) {
+        const uint32_t s = (uint32_t)__builtin_ctzll(past);
+        pos = __builtin_amdgcn_readlane(m, s);
+        count = s;
+        if (pos == kStop && s != 0) {
+            count = s - 1;
+            pos = __builtin_amdgcn_readlane(m, s - 1);
+        }
+    } else {
+        /* 64 members in the window: the table maps 32 links on */
+        const uint32_t m64 = jump[uniform32(__builtin_amdgcn_readlane(m, kWave / 2))];
+        count = kWave;
+        pos = m64;
+        if (m64 == kStop) {
+            count = kWave - 1;
+            pos = __builtin_amdgcn_readlane(m, kWave - 1);
+        }
+    }
+    member = m;
+    count_out = uniform32(count);
+    return uniform32(pos);
+}
+
 /* What parsing one window decided. kGroup: execute `g` (staged at `delta`
  * in the stage buffer) and advance the source by `consumed`. kSlow: one
  * sequence the window cannot hold, by the scalar ladder (`q`, with the
@@ -781,7 +908,7 @@ struct Parsed {
  * (src/lz4_block.h:95-258 for every member, in parallel) and advances
  * sp and dp past what it decided. Reads the source only. */
 template <uint32_t kC>
-__device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
+__device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, gcu8* src,
                                                uint64_t src_size, uint64_t cap, uint64_t& sp,
                                                uint64_t& dp, uint32_t lane) {
     Parsed out;
@@ -831,7 +958,11 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, gcu8* src,
     /* walk the true chain */
     uint32_t count = 0;
     uint32_t member = 0; /* lane j: position of member j */
-    const uint32_t pos = walk_chain<kC>(link, start, count, member);
+    /* a one-wave window doubles (the wave's own latency is the pace); the
+     * two-wave parser walks, leaving LDS and issue to the copier */
+    const uint32_t pos = kC <= kCandSplit
+                             ? walk_doubling<kC>(link, start, jump, count, member, lane)
+                             : walk_chain<kC>(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
@@ -1063,7 +1194,7 @@ __device__ __forceinline__ Cand parse_candidate_snappy(const uint8_t* cw, uint32
  * is exactly the end of the source ends the chunk when the output so far
  * equals the declared length. */
 template <uint32_t kC>
-__device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
+__device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* jump, gcu8* src,
                                                       uint64_t src_size, uint64_t cap,
                                                       uint64_t declared, uint64_t& sp,
                                                       uint64_t& dp, uint32_t lane) {
@@ -1119,7 +1250,11 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
     }
     uint32_t count = 0;
     uint32_t member = 0;
-    const uint32_t pos = walk_chain<kC>(link, start, count, member);
+    /* a one-wave window doubles (the wave's own latency is the pace); the
+     * two-wave parser walks, leaving LDS and issue to the copier */
+    const uint32_t pos = kC <= kCandSplit
+                             ? walk_doubling<kC>(link, start, jump, count, member, lane)
+                             : walk_chain<kC>(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
@@ -1244,13 +1379,14 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, gcu8* src,
 
 /* A window of either format. */
 template <int kFormat, uint32_t kC>
-__device__ __forceinline__ Parsed parse_any(uint8_t* stage, gcu8* src, uint64_t src_size,
+__device__ __forceinline__ Parsed parse_any(uint8_t* stage, uint16_t* jump, gcu8* src,
+                                            uint64_t src_size,
                                             uint64_t cap, uint64_t declared, uint64_t& sp,
                                             uint64_t& dp, uint32_t lane) {
     if (kFormat == kFormatSnappy) {
-        return parse_window_snappy<kC>(stage, src, src_size, cap, declared, sp, dp, lane);
+        return parse_window_snappy<kC>(stage, jump, src, src_size, cap, declared, sp, dp, lane);
     }
-    return parse_window<kC>(stage, src, src_size, cap, sp, dp, lane);
+    return parse_window<kC>(stage, jump, src, src_size, cap, sp, dp, lane);
 }
 
 /* Start of a chunk: the Snappy preamble and its declared length against the
@@ -1272,7 +1408,8 @@ __device__ __forceinline__ int chunk_begin(gcu8* src, uint64_t src_size, uint64_
 
 /* Executes a sequence the window could not hold: everything in the ring
  * goes out first, then the literal and the match run global to global. */
-__device__ __forceinline__ void execute_slow(WaveLds& w, gcu8* src, gu8* dst,
+template <class W>
+__device__ __forceinline__ void execute_slow(W& w, gcu8* src, gu8* dst,
                                              uint64_t cap, const Lz4Step& q, uint64_t dp_after,
                                              uint64_t& dp,
                                              uint64_t& flushed, uint64_t& ring_lo,
@@ -1305,14 +1442,21 @@ struct alignas(16) PipeRec {
     uint32_t terminal;
     uint32_t end_rel;
     uint32_t lead;              /* kGroup: the slow sequence below runs first */
+    uint32_t placed;            /* kGroup: the parser put the literals in the ring */
     uint64_t lit_src, lit_dst, lit_len_s, match_dst, offset_s, match_len_s, dp_after;
 };
 
 struct alignas(16) PipeLds {
-    WaveLds w;                  /* ring (copier) and stage buffer 0 */
-    WaveLds w2;                 /* the second wave's own, once the block splits */
-    uint8_t stage1[kStageBuf];  /* stage buffer 1 */
-    PipeRec rec[2];
+    union {
+        struct {
+            PipeRing w;                 /* copier's ring, stage buffer 0, links */
+            uint8_t stage1[kStageBuf];  /* stage buffer 1 */
+            PipeRec rec[2];
+        } pipe;                         /* two-wave form */
+        struct {
+            WaveLds a, b;               /* one wave each, once the block splits */
+        } split;
+    };
     uint32_t chunk_idx;
     uint32_t fin;               /* step whose record ends the chunk */
     uint32_t pad[2];
@@ -1342,7 +1486,8 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
             break;
         }
         const Parsed pr =
-            parse_any<kFormat, kCandSplit>(w.stage, src, src_size, cap, declared, sp, pdp, lane);
+            parse_any<kFormat, kCandSplit>(w.stage, w.jump, src, src_size, cap, declared, sp, pdp,
+                                           lane);
         if (pr.kind == kParseFail) {
             status = pr.status;
             break;
@@ -1488,11 +1633,18 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
         /* the parser's preamble verdict (Snappy), handed over as a failed
          * window record when it is not OK */
         const int begun = parser ? chunk_begin<kFormat>(src, src_size, cap, sp, declared) : kOk;
+        /* LZ4 windows have the parser put their literals in the ring (the
+         * copier is the busier wave there); Snappy's parser is the busier
+         * one, so its copier keeps them. The record being executed
+         * meanwhile has a slow sequence: that stages through the ring, so
+         * this window's literals stay out. */
+        constexpr bool kPlace = kFormat != kFormatSnappy;
+        bool prev_slow = false;
         for (uint32_t step = 0;; step++) {
             const uint32_t b = step & 1;
             if (parser) {
                 if (!pdone) {
-                    PipeRec& r = L.rec[b];
+                    PipeRec& r = L.pipe.rec[b];
                     Parsed pr;
                     if (begun != kOk || fuel-- == 0) {
                         pr.kind = kParseFail;
@@ -1500,7 +1652,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         pr.done = false;
                         pr.delta = 0;
                     } else {
-                        pr = parse_any<kFormat, kCandPipe>(b ? L.stage1 : L.w.stage, src,
+                        pr = parse_any<kFormat, kCandPipe>(b ? L.pipe.stage1 : L.pipe.w.stage,
+                                                           nullptr, src,
                                                            src_size, cap,
                                                 declared, sp, dp, lane);
                     }
@@ -1518,6 +1671,16 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         r.dp_after = pr.lead ? pr.lead_dp : dp;
                     }
                     if (pr.kind == kParseGroup) {
+                        /* the copier works below this group's start meanwhile
+                         * (and reads no history within kBatchMax of it) */
+                        const bool place = kPlace && !pr.lead && !prev_slow;
+                        if (place) {
+                            const uint8_t* cw =
+                                (b ? L.pipe.stage1 : L.pipe.w.stage) + pr.delta;
+                            place_literals(L.pipe.w, cw, pr.g, (uintptr_t)dst,
+                                           dp - pr.g.end_rel, lane);
+                        }
+                        r.placed = place ? 1u : 0u;
                         r.lit[lane] = (uint16_t)pr.g.lit;
                         r.lit_len[lane] = (uint16_t)pr.g.lit_len;
                         r.match_len[lane] = (uint16_t)pr.g.match_len;
@@ -1528,13 +1691,14 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         r.terminal = pr.g.terminal ? 1u : 0u;
                         r.end_rel = pr.g.end_rel;
                     }
+                    prev_slow = pr.kind != kParseGroup || pr.lead;
                     pdone = pr.kind == kParseFail || pr.done;
                     if (pdone) {
                         L.fin = step;
                     }
                 }
             } else if (step > 0 && status == kOk) {
-                const PipeRec& r = L.rec[b ^ 1];
+                const PipeRec& r = L.pipe.rec[b ^ 1];
                 const uint32_t kind = uniform32(r.kind);
                 if (kind == kParseFail) {
                     status = (int)uniform32((uint32_t)r.status);
@@ -1547,7 +1711,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         s.match_dst = uniform64(r.match_dst);
                         s.offset = (uint32_t)uniform64(r.offset_s);
                         s.match_len = uniform64(r.match_len_s);
-                        execute_slow(L.w, src, dst, cap, s, uniform64(r.dp_after), dp, flushed,
+                        execute_slow(L.pipe.w, src, dst, cap, s, uniform64(r.dp_after), dp, flushed,
                                      ring_lo, lane);
                     }
                     if (kind == kParseGroup) {
@@ -1560,9 +1724,11 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         g.take = uniform32(r.take);
                         g.terminal = uniform32(r.terminal) != 0;
                         g.end_rel = uniform32(r.end_rel);
-                        const uint8_t* cw = (b ? L.w.stage : L.stage1) + uniform32(r.delta);
-                        execute_group(L.w, cw, g, dst, cap, dp, flushed, ring_lo,
-                                      uniform32(r.done) != 0, lane);
+                        const uint8_t* cw =
+                            (b ? L.pipe.w.stage : L.pipe.stage1) + uniform32(r.delta);
+                        execute_group(L.pipe.w, cw, g, dst, cap, dp, flushed, ring_lo,
+                                      uniform32(r.done) != 0, lane, uniform32(r.placed) != 0,
+                                      kPlace ? kBatchMax : 0);
                     }
                 }
             }
@@ -1581,7 +1747,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
     }
     /* one wave per chunk from here: the parser decodes the chunk in hand,
      * the other wave takes its own from the queue; no barrier follows */
-    WaveLds& w = parser ? L.w : L.w2;
+    WaveLds& w = parser ? L.split.a : L.split.b;
     uint32_t idx = parser ? split_idx : uniform32(atomicAdd(&q->next, lane == 0 ? 1u : 0u));
     while (idx < n) {
         const uint64_t chunk = order[idx];
