@@ -85,6 +85,7 @@ struct alignas(16) RingLds {
     static constexpr uint32_t kSize = kR;
     uint8_t ring[kR];
     uint8_t stage[kStageBuf];
+    uint8_t sink[kWave]; /* stores of lanes with nothing to store */
 };
 struct alignas(16) WaveLds : RingLds<kRing> {
     uint16_t jump[kCandSplit * kWave];
@@ -565,6 +566,16 @@ __device__ __forceinline__ void wave_match_global(W& w, gu8* dst, uint64_t m,
 
 /* ------------------------------------------------- group execution */
 
+/* lane mod d for lane < 64 and d >= 1: the float quotient lane * (1 / d)
+ * never exceeds the true one (the reciprocal is within one ulp) and falls
+ * short of it by at most one, at exact multiples, so one correction
+ * suffices. For d > 64 the quotient is 0, so a 24-bit product is exact. */
+__device__ __forceinline__ uint32_t lane_mod(uint32_t lane, float lane_f, uint32_t d) {
+    const float inv = __builtin_amdgcn_rcpf((float)d);
+    const uint32_t r = lane - __umul24(d, (uint32_t)(lane_f * inv));
+    return r >= d ? r - d : r;
+}
+
 /* The members of one window that a group executes, as the copy phase needs
  * them: per lane, the member's literal start (window-relative), literal
  * and match lengths, offset and output offset from the group start; and,
@@ -705,38 +716,39 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
             const uint32_t rm = ja & 8191;
             const uint32_t rs = (ja >> 13) & 8191;
             const uint32_t jm = (ja >> 26) + 1;
-            uint32_t r = lane;
-            if (lane >= jo) {
-                const float inv = __builtin_amdgcn_rcpf((float)jo);
-                r = lane - __umul24(jo, (uint32_t)(lane_f * inv));
-                r = r >= jo ? r - jo : r;
-            }
-            if (lane < jm) {
-                w.ring[(rm + lane) & (W::kSize - 1)] = w.ring[(rs + r) & (W::kSize - 1)];
+            /* lane mod jo (lanes below jo get a zero quotient); lanes past
+             * the match write their own sink byte, so no lane is masked */
+            const uint32_t r = lane_mod(lane, lane_f, jo);
+            uint8_t* at = lane < jm ? &w.ring[(rm + lane) & (W::kSize - 1)] : &w.sink[lane];
+            *at = w.ring[(rs + r) & (W::kSize - 1)];
+        } else {
+            /* a wide literal (unless placed), then the match unless the
+             * member is the terminal literal (no rest member is a lane
+             * match) */
+            const uint32_t jl = __builtin_amdgcn_readlane(lit_len, j);
+            const uint32_t jlit = __builtin_amdgcn_readlane(lit, j);
+            const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
+            const uint64_t jd0 = D0 + jd;
+            if (!placed && jl > kLaneLit) {
+                for (uint32_t i = lane; i < jl; i += kWave) {
+                    w.ring[ring_index<W::kSize>(da, jd0 + i)] = cw[jlit + i];
+                }
             }
             wave_order();
-            continue;
-        }
-        const uint32_t jl = __builtin_amdgcn_readlane(lit_len, j);
-        const uint32_t jlit = __builtin_amdgcn_readlane(lit, j);
-        const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
-        const uint64_t jd0 = D0 + jd;
-        if (!placed && jl > kLaneLit) {
-            for (uint32_t i = lane; i < jl; i += kWave) {
-                w.ring[ring_index<W::kSize>(da, jd0 + i)] = cw[jlit + i];
-            }
-        }
-        wave_order();
-        if (j < take) {
-            const uint32_t jm = __builtin_amdgcn_readlane(match_len, j);
-            const uint32_t joff = __builtin_amdgcn_readlane(offset, j);
-            const bool jdone = __builtin_amdgcn_readlane((uint32_t)lane_match, j) != 0;
-            if (!jdone) {
+            if (j < take) {
+                const uint32_t jm = __builtin_amdgcn_readlane(match_len, j);
+                const uint32_t joff = __builtin_amdgcn_readlane(offset, j);
                 const uint64_t jm0 = jd0 + jl;
                 const uint64_t jfrom = jm0 - joff;
+                /* byte i copies period byte i mod joff, kept per lane as i
+                 * steps by 64 */
+                const uint32_t step = kWave % joff;
+                uint32_t r = lane_mod(lane, lane_f, joff);
                 for (uint32_t i = lane; i < jm; i += kWave) {
-                    const uint64_t yy = jfrom + (i < joff ? i : i % joff);
-                    w.ring[ring_index<W::kSize>(da, jm0 + i)] = history_byte(w, dst, yy, hist_lo);
+                    w.ring[ring_index<W::kSize>(da, jm0 + i)] =
+                        history_byte(w, dst, jfrom + r, hist_lo);
+                    r += step;
+                    r = r >= joff ? r - joff : r;
                 }
             }
         }
@@ -775,7 +787,9 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
         /* Per member: read the link of pos out of lane rel, record pos as
          * member `count`, step. The loop leaves once pos is outside this
          * region (a stop link lands far outside) or 64 members are held.
-         * Only scalar registers are written apart from `member`. */
+         * Only scalar registers are written apart from `member`. Two
+         * members a turn, the current position alternating between pos and
+         * nx, so that no step moves it. */
         uint32_t prev, nx, t;
         /* The count lives in m0 during the loop (saved and restored around
          * it): v_writelane takes its value and lane select from scalar
@@ -791,13 +805,24 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
             "walk_loop_%=:\n\t"
             "v_readlane_b32 %[nx], %[lk], %[rel]\n\t"
             "v_writelane_b32 %[mem], %[pos], m0\n\t"
-            "s_mov_b32 %[prev], %[pos]\n\t"
-            "s_mov_b32 %[pos], %[nx]\n\t"
             "s_sub_u32 %[rel], %[nx], %[base]\n\t"
             "s_add_u32 m0, m0, 1\n\t"
             "s_or_b32 %[t], %[rel], m0\n\t"
             "s_cmp_lt_u32 %[t], 64\n\t"
+            "s_cbranch_scc0 walk_out_nx_%=\n\t"
+            "v_readlane_b32 %[pos], %[lk], %[rel]\n\t"
+            "v_writelane_b32 %[mem], %[nx], m0\n\t"
+            "s_sub_u32 %[rel], %[pos], %[base]\n\t"
+            "s_add_u32 m0, m0, 1\n\t"
+            "s_or_b32 %[t], %[rel], m0\n\t"
+            "s_cmp_lt_u32 %[t], 64\n\t"
             "s_cbranch_scc1 walk_loop_%=\n\t"
+            "s_mov_b32 %[prev], %[nx]\n\t"
+            "s_branch walk_done_%=\n"
+            "walk_out_nx_%=:\n\t"
+            "s_mov_b32 %[prev], %[pos]\n\t"
+            "s_mov_b32 %[pos], %[nx]\n"
+            "walk_done_%=:\n\t"
             "s_mov_b32 %[cnt], m0\n\t"
             "s_mov_b32 m0, %[m0s]"
             : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+s"(count), [rel] "+s"(rel),
