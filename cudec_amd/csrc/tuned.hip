@@ -64,7 +64,8 @@ constexpr uint32_t kBatchMax = 2048;      /* output bytes of one batch */
 constexpr uint32_t kStop = 0xFFFFu;       /* link of an unresolvable candidate */
 constexpr uint32_t kLaneLit = 64;         /* literal runs copied by their lane */
 constexpr uint32_t kLaneMatch = 64;       /* matches copied by their lane (<= 64) */
-constexpr int kBins = 256;
+constexpr uint32_t kOctaveBits = 2;       /* cost bins per octave: 4 */
+constexpr int kBins = 64 << kOctaveBits;
 /* Default: chunks costing at least 512 KiB (capacity + capacity - compressed) run
  * on two waves. */
 constexpr uint64_t kDefaultPipeMinCost = 512u << 10;
@@ -93,13 +94,30 @@ struct alignas(16) WaveLds : RingLds<kRing> {
 constexpr uint32_t kPipeRing = 8192;
 using PipeRing = RingLds<kPipeRing>;
 
+/* The decode queue hands out positions of the cost-sorted chunk order in
+ * up to kSubQueues interleaved sub-queues (of k: sub-queue s holds
+ * positions s, s + k, ...), each counter on its own 4 KiB page: the
+ * counters' atomics are served in parallel, where one counter for the
+ * whole grid serialises every chunk's pop (about 12 ns each, the pace of a
+ * batch of many small chunks). A batch of few chunks per wave keeps one
+ * queue: its exact costliest-first order matters more there. */
+constexpr uint32_t kSubQueues = 8;
+constexpr uint32_t kSubStride = 1024; /* uint32 words: 4 KiB apart */
+constexpr uint64_t kSubQueueMinPerWave = 16; /* chunks per wave for kSubQueues */
+
+/* A wave's place in the decode queue: the sub-queue it draws from and how
+ * many sub-queues it has not yet found drained. */
+struct QueueCursor {
+    uint32_t sub;
+    uint32_t left;
+};
+
 /* Scratch of one tuned launch (stream-ordered allocation). */
 struct Queue {
+    uint32_t next[kSubQueues * kSubStride];
     uint32_t bin_count[kBins];
     uint32_t bin_base[kBins];
     uint32_t bin_cursor[kBins];
-    uint32_t next;
-    uint32_t pad[3];
 };
 
 template <uint32_t kR = kRing>
@@ -1544,9 +1562,21 @@ __global__ void queue_reset(Queue* q) {
         q->bin_count[t] = 0;
         q->bin_cursor[t] = 0;
     }
-    if (t == 0) {
-        q->next = 0;
+    if (t < kSubQueues) {
+        q->next[t * kSubStride] = 0;
     }
+}
+
+/* Order-preserving key of a cost below 2^63: the octave and the next
+ * kOctaveBits bits below its leading one (costs below 2^kOctaveBits are
+ * their own key). */
+__host__ __device__ inline uint32_t cost_key(uint64_t cost) {
+    if (cost < (1u << kOctaveBits)) {
+        return (uint32_t)cost;
+    }
+    const uint32_t lg = 63u - (uint32_t)__builtin_clzll(cost);
+    return lg << kOctaveBits |
+           (uint32_t)((cost >> (lg - kOctaveBits)) & ((1u << kOctaveBits) - 1));
 }
 
 /* Bin of a chunk by its decode cost, compressed size plus capacity: a
@@ -1561,18 +1591,27 @@ __device__ __forceinline__ uint32_t cost_bin(uint64_t src_size, uint64_t cap) {
     /* the output, plus what matches add to it (capacity - compressed):
      * match bytes cost the decode its time, literal bytes are a copy */
     const uint64_t cost = b + (b > a ? b - a : 0);
-    if (cost < 4) {
-        return kBins - 1 - (uint32_t)cost;
-    }
-    const uint32_t lg = 63u - (uint32_t)__builtin_clzll(cost);   /* 2..62 */
-    const uint32_t key = lg * 4 + (uint32_t)((cost >> (lg - 2)) & 3);
-    return kBins - 1 - key;
+    return kBins - 1 - cost_key(cost);
 }
 
+/* Chunks per cost bin. A block counts its chunks in LDS and adds one
+ * global atomic per bin it holds: a batch of like chunks (one bin) would
+ * otherwise serialise one atomic per chunk on that bin's counter. */
 __global__ void queue_count(const size_t* sizes, const size_t* caps, size_t n, Queue* q) {
+    __shared__ uint32_t local[kBins];
+    for (uint32_t t = threadIdx.x; t < kBins; t += blockDim.x) {
+        local[t] = 0;
+    }
+    __syncthreads();
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        atomicAdd(&q->bin_count[cost_bin(sizes[i], caps[i])], 1u);
+        atomicAdd(&local[cost_bin(sizes[i], caps[i])], 1u);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kBins; t += blockDim.x) {
+        if (local[t] != 0) {
+            atomicAdd(&q->bin_count[t], local[t]);
+        }
     }
 }
 
@@ -1586,13 +1625,58 @@ __global__ void queue_scan(Queue* q) {
     }
 }
 
+/* Each chunk's place in the order: its bin's base, the block's range in
+ * that bin (one global atomic per bin the block holds) and its rank in the
+ * block (an LDS atomic). */
 __global__ void queue_fill(const size_t* sizes, const size_t* caps, size_t n, Queue* q,
                            uint32_t* order) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const uint32_t b = cost_bin(sizes[i], caps[i]);
-        order[q->bin_base[b] + atomicAdd(&q->bin_cursor[b], 1u)] = (uint32_t)i;
+    __shared__ uint32_t local[kBins];
+    __shared__ uint32_t range[kBins];
+    for (uint32_t t = threadIdx.x; t < kBins; t += blockDim.x) {
+        local[t] = 0;
     }
+    __syncthreads();
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t b = 0, rank = 0;
+    if (i < n) {
+        b = cost_bin(sizes[i], caps[i]);
+        rank = atomicAdd(&local[b], 1u);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kBins; t += blockDim.x) {
+        if (local[t] != 0) {
+            range[t] = atomicAdd(&q->bin_cursor[t], local[t]);
+        }
+    }
+    __syncthreads();
+    if (i < n) {
+        order[q->bin_base[b] + range[b] + rank] = (uint32_t)i;
+    }
+}
+
+/* The next queue position for this wave from kSubs sub-queues (1: the
+ * single counter), or at least n once every one is drained (n < 2^32).
+ * Every lane takes part in the atomic and only lane 0 adds: a
+ * lane-0-only branch at a persistent loop's top and bottom lets the
+ * compiler fold both into an exec-masked inner loop that only lane 0 can
+ * leave. */
+template <uint32_t kSubs>
+__device__ __forceinline__ uint32_t queue_pop(Queue* q, size_t n, QueueCursor& c,
+                                              uint32_t lane) {
+    if (kSubs == 1) {
+        return uniform32(atomicAdd(&q->next[0], lane == 0 ? 1u : 0u));
+    }
+    while (c.left != 0) {
+        const uint32_t i =
+            uniform32(atomicAdd(&q->next[c.sub * kSubStride], lane == 0 ? 1u : 0u));
+        const uint64_t pos = (uint64_t)c.sub + (uint64_t)kSubs * i;
+        if (pos < n) {
+            return (uint32_t)pos;
+        }
+        c.sub = (c.sub + 1) % kSubs;
+        c.left--;
+    }
+    return (uint32_t)n;
 }
 
 constexpr int kPipeBlock = 2 * kWave;
@@ -1610,8 +1694,10 @@ constexpr uint32_t kNoFin = 0xFFFFFFFFu;
 /* kWaves: waves per SIMD the register budget is cut for. 4 (128 VGPRs) for
  * launches that pipeline chunks; 5 (96 VGPRs, some spills on the two-wave
  * path) for launches where every chunk runs on one wave, which are
- * throughput-bound and gain from a fifth wave. */
-template <int kFormat, int kWaves>
+ * throughput-bound and gain from a fifth wave. kSubs: decode sub-queues,
+ * kSubQueues for launches of many chunks per wave, else 1 (the single
+ * counter, whose code keeps the fewest scalar registers live). */
+template <int kFormat, int kWaves, uint32_t kSubs>
 __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu(kWaves)))
     piped_batch(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
                 const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
@@ -1620,6 +1706,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
     const uint32_t lane = __lane_id();
     const bool parser = uniform32(threadIdx.x / kWave) == 0;
     uint32_t split_idx = 0xFFFFFFFFu; /* first chunk decoded one wave per chunk */
+    QueueCursor cursor{blockIdx.x % kSubs, kSubs};
     /* pipe_bin: chunks whose cost bin is at most this run on two waves; -1:
      * none (the first chunk splits the block) */
     for (;;) {
@@ -1627,12 +1714,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
          * (chunk_idx, fin) before the parser writes the next chunk's */
         __syncthreads();
         if (parser) {
-            /* every lane takes part in the queue atomic and only lane 0 adds:
-             * a lane-0-only branch at a persistent loop's top and bottom lets
-             * the compiler fold both into an exec-masked inner loop that only
-             * lane 0 can leave */
-            const uint32_t mine = atomicAdd(&q->next, lane == 0 ? 1u : 0u);
-            L.chunk_idx = uniform32(mine);
+            L.chunk_idx = queue_pop<kSubs>(q, n, cursor, lane);
             L.fin = kNoFin;
         }
         __syncthreads();
@@ -1773,7 +1855,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
     /* one wave per chunk from here: the parser decodes the chunk in hand,
      * the other wave takes its own from the queue; no barrier follows */
     WaveLds& w = parser ? L.split.a : L.split.b;
-    uint32_t idx = parser ? split_idx : uniform32(atomicAdd(&q->next, lane == 0 ? 1u : 0u));
+    uint32_t idx = parser ? split_idx : queue_pop<kSubs>(q, n, cursor, lane);
     while (idx < n) {
         const uint64_t chunk = order[idx];
         gcu8* src = global_bytes(srcs[chunk]);
@@ -1782,7 +1864,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
         const uint64_t cap = uniform64(caps[chunk]);
         const ChunkOut o = decode_chunk<kFormat>(w, src, src_size, dst, cap, lane);
         write_result(results, chunk, (int)uniform32((uint32_t)o.status), uniform64(o.produced));
-        idx = uniform32(atomicAdd(&q->next, lane == 0 ? 1u : 0u));
+        idx = queue_pop<kSubs>(q, n, cursor, lane);
     }
 }
 
@@ -1809,12 +1891,7 @@ int32_t pipe_bin_for(uint64_t min_cost) {
     if (min_cost == UINT64_MAX) {
         return -1;
     }
-    const uint64_t c = min_cost;
-    if (c < 4) {
-        return (int32_t)(kBins - 1 - c);
-    }
-    const uint32_t lg = 63u - (uint32_t)__builtin_clzll(c);
-    return (int32_t)(kBins - 1 - (lg * 4 + (uint32_t)((c >> (lg - 2)) & 3)));
+    return (int32_t)(kBins - 1 - cost_key(min_cost));
 }
 DeviceState g_devices[kMaxDevices];
 std::mutex g_devices_lock;
@@ -1836,13 +1913,13 @@ hipError_t device_state(DeviceState** out) {
             return e;
         }
         int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 4>,
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 4, 1>,
                                                          kPipeBlock, 0);
         if (e != hipSuccess) {
             return e;
         }
         d.pipe_blocks_per_cu = per > 0 ? per : 1;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 5>,
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 5, 1>,
                                                          kPipeBlock, 0);
         if (e != hipSuccess) {
             return e;
@@ -1944,22 +2021,23 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
         grid = n;
     }
     const dim3 gd((unsigned)grid), bd(kPipeBlock);
-    if (format == kFormatSnappy) {
-        if (all_split) {
-            hipLaunchKernelGGL((piped_batch<kFormatSnappy, 5>), gd, bd, 0, stream, srcs, src_sizes,
-                               dsts, caps, n, results, q, order, pipe_bin);
-        } else {
-            hipLaunchKernelGGL((piped_batch<kFormatSnappy, 4>), gd, bd, 0, stream, srcs, src_sizes,
-                               dsts, caps, n, results, q, order, pipe_bin);
-        }
-    } else {
-        if (all_split) {
-            hipLaunchKernelGGL((piped_batch<kFormatLz4, 5>), gd, bd, 0, stream, srcs, src_sizes,
-                               dsts, caps, n, results, q, order, pipe_bin);
-        } else {
-            hipLaunchKernelGGL((piped_batch<kFormatLz4, 4>), gd, bd, 0, stream, srcs, src_sizes,
-                               dsts, caps, n, results, q, order, pipe_bin);
-        }
+    const bool many = (uint64_t)n >= kSubQueueMinPerWave * 2 * (uint64_t)grid;
+    const int shape = (format == kFormatSnappy ? 4 : 0) | (all_split ? 2 : 0) | (many ? 1 : 0);
+    switch (shape) {
+#define CUDEC_PIPED(kShape, kFmt, kW, kS)                                                   \
+    case kShape:                                                                            \
+        hipLaunchKernelGGL((piped_batch<kFmt, kW, kS>), gd, bd, 0, stream, srcs, src_sizes, \
+                           dsts, caps, n, results, q, order, pipe_bin);                     \
+        break;
+        CUDEC_PIPED(0, kFormatLz4, 4, 1)
+        CUDEC_PIPED(1, kFormatLz4, 4, kSubQueues)
+        CUDEC_PIPED(2, kFormatLz4, 5, 1)
+        CUDEC_PIPED(3, kFormatLz4, 5, kSubQueues)
+        CUDEC_PIPED(4, kFormatSnappy, 4, 1)
+        CUDEC_PIPED(5, kFormatSnappy, 4, kSubQueues)
+        CUDEC_PIPED(6, kFormatSnappy, 5, 1)
+        CUDEC_PIPED(7, kFormatSnappy, 5, kSubQueues)
+#undef CUDEC_PIPED
     }
     e = hipGetLastError();
     const hipError_t f = hipFreeAsync(scratch, stream);
