@@ -49,12 +49,16 @@ namespace cudec_amd {
 namespace {
 
 /* Candidates per lane: a window spans kC * 64 candidate positions. The
- * two-wave form parses windows twice as wide as the one-wave form: its
- * parser runs beside the copier, and a wider window fills more lanes of
- * every lane-parallel step, while a lone wave pays the wider speculation
- * on its own critical path. */
+ * two-wave form parses wider windows than the one-wave form: its parser
+ * runs beside the copier, and a wider window fills more lanes of every
+ * lane-parallel step, while a lone wave pays the wider speculation on its
+ * own critical path. Snappy's two-wave windows are 768 candidates: with
+ * them its two-wave path fits 96 VGPRs, so Snappy launches run five waves
+ * per SIMD throughout (1024-candidate LZ4 windows at four waves per SIMD
+ * are as fast as 768 at five). */
 constexpr uint32_t kCandSplit = 8;
 constexpr uint32_t kCandPipe = 16;
+constexpr uint32_t kCandPipeSnappy = 12;
 /* staged bytes a window of kC candidates per lane reads (the window plus
  * the longest candidate that starts in it) */
 constexpr uint32_t stage_bytes(uint32_t kc) { return kc * 64u + 512u; }
@@ -1759,10 +1763,10 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         pr.done = false;
                         pr.delta = 0;
                     } else {
-                        pr = parse_any<kFormat, kCandPipe>(b ? L.pipe.stage1 : L.pipe.w.stage,
-                                                           nullptr, src,
-                                                           src_size, cap,
-                                                declared, sp, dp, lane);
+                        constexpr uint32_t kC =
+                            kFormat == kFormatSnappy ? kCandPipeSnappy : kCandPipe;
+                        pr = parse_any<kFormat, kC>(b ? L.pipe.stage1 : L.pipe.w.stage, nullptr,
+                                                    src, src_size, cap, declared, sp, dp, lane);
                     }
                     r.kind = pr.kind;
                     r.status = pr.status;
@@ -1874,7 +1878,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
 struct DeviceState {
     int cus = 0;
     int pipe_blocks_per_cu = 0;  /* piped_batch<_, 4> (two-wave blocks) */
-    int split_blocks_per_cu = 0; /* piped_batch<_, 5>, launches with no two-wave chunk */
+    int split_blocks_per_cu = 0; /* piped_batch<_, 5>: no two-wave chunk, or Snappy */
     uint64_t lz4_pipe_min_cost = kDefaultPipeMinCost; /* two waves at or above this cost */
     hipMemPool_t pool = nullptr;
     bool ready = false;
@@ -2012,17 +2016,18 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q, order);
     const int32_t pipe_bin = pipe_bin_for(min_cost);
     /* two-wave blocks; a block splits into two one-wave decoders when it
-     * meets a chunk below the threshold. With no two-wave chunk at all the
-     * five-wave build runs (every block splits at once). */
-    const bool all_split = pipe_bin < 0;
+     * meets a chunk below the threshold. The five-wave build runs where no
+     * chunk is two-wave (every block splits at once) and for Snappy, whose
+     * two-wave path fits its 96 VGPRs. */
+    const bool five_waves = pipe_bin < 0 || format == kFormatSnappy;
     size_t grid = (size_t)dev->cus *
-                  (size_t)(all_split ? dev->split_blocks_per_cu : dev->pipe_blocks_per_cu);
+                  (size_t)(five_waves ? dev->split_blocks_per_cu : dev->pipe_blocks_per_cu);
     if (grid > n) {
         grid = n;
     }
     const dim3 gd((unsigned)grid), bd(kPipeBlock);
     const bool many = (uint64_t)n >= kSubQueueMinPerWave * 2 * (uint64_t)grid;
-    const int shape = (format == kFormatSnappy ? 4 : 0) | (all_split ? 2 : 0) | (many ? 1 : 0);
+    const int shape = (format == kFormatSnappy ? 4 : 0) | (five_waves ? 2 : 0) | (many ? 1 : 0);
     switch (shape) {
 #define CUDEC_PIPED(kShape, kFmt, kW, kS)                                                   \
     case kShape:                                                                            \
@@ -2033,11 +2038,11 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
         CUDEC_PIPED(1, kFormatLz4, 4, kSubQueues)
         CUDEC_PIPED(2, kFormatLz4, 5, 1)
         CUDEC_PIPED(3, kFormatLz4, 5, kSubQueues)
-        CUDEC_PIPED(4, kFormatSnappy, 4, 1)
-        CUDEC_PIPED(5, kFormatSnappy, 4, kSubQueues)
         CUDEC_PIPED(6, kFormatSnappy, 5, 1)
         CUDEC_PIPED(7, kFormatSnappy, 5, kSubQueues)
 #undef CUDEC_PIPED
+        default: /* Snappy always takes the five-wave build */
+            return hipErrorInvalidValue;
     }
     e = hipGetLastError();
     const hipError_t f = hipFreeAsync(scratch, stream);
