@@ -588,6 +588,11 @@ __device__ __forceinline__ void wave_match_global(W& w, gu8* dst, uint64_t m,
 
 /* ------------------------------------------------- group execution */
 
+/* Clears bit j of a wave-uniform mask in one scalar instruction. */
+__device__ __forceinline__ void clear_bit(uint64_t& mask, uint32_t j) {
+    asm("s_bitset0_b64 %0, %1" : "+s"(mask) : "s"(j));
+}
+
 /* lane mod d for lane < 64 and d >= 1: the float quotient lane * (1 / d)
  * never exceeds the true one (the reciprocal is within one ulp) and falls
  * short of it by at most one, at exact multiples, so one correction
@@ -716,25 +721,31 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
      * short match (<= 64 bytes) whose period is in the ring, is described by
      * its lane up front in two words, so the loop pays two v_readlane and a
      * byte a lane for it: fast_a = ring index of the match (13 bits), of
-     * its period (13 bits), length - 1 (6 bits); fast_b = min(offset, 64)
-     * (7 bits) and the flag (bit 31). Byte i copies period byte i mod
-     * offset (i < 64; the float quotient is exact after one correction),
-     * which precedes the match, so the lanes do not depend on each other. */
-    const bool rest_fast = full && !lane_match && !wide_lit && match_len != 0 && match_len <= kWave &&
-                           y >= hist_lo;
+     * its period (13 bits), length - 1 (6 bits); fast_b = min(offset, 64).
+     * Byte i copies period byte i mod offset (i < 64; the float quotient
+     * is exact after one correction), which precedes the match, so the
+     * lanes do not depend on each other. */
+    const bool rest_fast = full && !lane_match && !wide_lit && match_len != 0 &&
+                           match_len <= kWave && y >= hist_lo;
     static_assert(W::kSize <= 8192, "ring indices are packed in 13 bits");
     const uint32_t fast_a = ring_index<W::kSize>(da, m_at) | ring_index<W::kSize>(da, y) << 13 |
                             (match_len - 1) << 26;
-    const uint32_t fast_b = (offset < kWave ? offset : kWave) | (rest_fast ? 1u << 31 : 0u);
+    const uint32_t fast_b = offset < kWave ? offset : kWave;
     const float lane_f = (float)lane;
-    uint64_t rest = __ballot(active && (wide_lit || (full && !lane_match)));
-    while (rest != 0) {
-        const uint32_t j = __builtin_ctzll(rest);
-        rest &= rest - 1;
-        const uint32_t jb = __builtin_amdgcn_readlane(fast_b, j);
-        if (jb >> 31) {
+    /* in stream order: runs of short ring matches (one straight-line step
+     * each, in a loop with no other branch), each run ended by the next
+     * other member */
+    uint64_t fast = __ballot(rest_fast);
+    uint64_t other = __ballot(active && (wide_lit || (full && !lane_match))) & ~fast;
+    for (;;) {
+        /* the fast members before the next other one */
+        uint64_t run = other != 0 ? fast & ((other & (0 - other)) - 1) : fast;
+        fast &= ~run;
+        while (run != 0) {
+            const uint32_t j = __builtin_ctzll(run);
+            clear_bit(run, j);
             const uint32_t ja = __builtin_amdgcn_readlane(fast_a, j);
-            const uint32_t jo = jb & 127;
+            const uint32_t jo = __builtin_amdgcn_readlane(fast_b, j);
             const uint32_t rm = ja & 8191;
             const uint32_t rs = (ja >> 13) & 8191;
             const uint32_t jm = (ja >> 26) + 1;
@@ -743,35 +754,39 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
             const uint32_t r = lane_mod(lane, lane_f, jo);
             uint8_t* at = lane < jm ? &w.ring[(rm + lane) & (W::kSize - 1)] : &w.sink[lane];
             *at = w.ring[(rs + r) & (W::kSize - 1)];
-        } else {
-            /* a wide literal (unless placed), then the match unless the
-             * member is the terminal literal (no rest member is a lane
-             * match) */
-            const uint32_t jl = __builtin_amdgcn_readlane(lit_len, j);
-            const uint32_t jlit = __builtin_amdgcn_readlane(lit, j);
-            const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
-            const uint64_t jd0 = D0 + jd;
-            if (!placed && jl > kLaneLit) {
-                for (uint32_t i = lane; i < jl; i += kWave) {
-                    w.ring[ring_index<W::kSize>(da, jd0 + i)] = cw[jlit + i];
-                }
-            }
             wave_order();
-            if (j < take) {
-                const uint32_t jm = __builtin_amdgcn_readlane(match_len, j);
-                const uint32_t joff = __builtin_amdgcn_readlane(offset, j);
-                const uint64_t jm0 = jd0 + jl;
-                const uint64_t jfrom = jm0 - joff;
-                /* byte i copies period byte i mod joff, kept per lane as i
-                 * steps by 64 */
-                const uint32_t step = kWave % joff;
-                uint32_t r = lane_mod(lane, lane_f, joff);
-                for (uint32_t i = lane; i < jm; i += kWave) {
-                    w.ring[ring_index<W::kSize>(da, jm0 + i)] =
-                        history_byte(w, dst, jfrom + r, hist_lo);
-                    r += step;
-                    r = r >= joff ? r - joff : r;
-                }
+        }
+        if (other == 0) {
+            break;
+        }
+        /* a wide literal (unless placed), then the match unless the member
+         * is the terminal literal (no rest member is a lane match) */
+        const uint32_t j = __builtin_ctzll(other);
+        clear_bit(other, j);
+        const uint32_t jl = __builtin_amdgcn_readlane(lit_len, j);
+        const uint32_t jlit = __builtin_amdgcn_readlane(lit, j);
+        const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
+        const uint64_t jd0 = D0 + jd;
+        if (!placed && jl > kLaneLit) {
+            for (uint32_t i = lane; i < jl; i += kWave) {
+                w.ring[ring_index<W::kSize>(da, jd0 + i)] = cw[jlit + i];
+            }
+        }
+        wave_order();
+        if (j < take) {
+            const uint32_t jm = __builtin_amdgcn_readlane(match_len, j);
+            const uint32_t joff = __builtin_amdgcn_readlane(offset, j);
+            const uint64_t jm0 = jd0 + jl;
+            const uint64_t jfrom = jm0 - joff;
+            /* byte i copies period byte i mod joff, kept per lane as i
+             * steps by 64 */
+            const uint32_t step = kWave % joff;
+            uint32_t r = lane_mod(lane, lane_f, joff);
+            for (uint32_t i = lane; i < jm; i += kWave) {
+                w.ring[ring_index<W::kSize>(da, jm0 + i)] =
+                    history_byte(w, dst, jfrom + r, hist_lo);
+                r += step;
+                r = r >= joff ? r - joff : r;
             }
         }
         wave_order();
