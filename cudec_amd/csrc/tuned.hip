@@ -641,45 +641,14 @@ __device__ __forceinline__ void put_literal(W& w, const uint8_t* cw, uint32_t li
     }
 }
 
-/* Every literal of group g (window staged at cw, group output starting at
- * D0) to its place in the ring: short ones by their lane, longer ones by
- * the whole wave. The two-wave form runs this on the parsing wave for the
- * window after the one being executed, so that the copying wave finds the
- * literals in place. */
-template <class W>
-__device__ __forceinline__ void place_literals(W& w, const uint8_t* cw, const Group& g,
-                                               uintptr_t da, uint64_t D0, uint32_t lane) {
-    const bool active = lane < g.take || (g.terminal && lane == g.take);
-    const bool wide_lit = g.lit_len > kLaneLit;
-    if (active && !wide_lit) {
-        put_literal(w, cw, g.lit, g.lit_len, da, D0 + g.drel);
-    }
-    uint64_t wide = __ballot(active && wide_lit);
-    while (wide != 0) {
-        const uint32_t j = __builtin_ctzll(wide);
-        wide &= wide - 1;
-        const uint32_t jl = __builtin_amdgcn_readlane(g.lit_len, j);
-        const uint32_t jlit = __builtin_amdgcn_readlane(g.lit, j);
-        const uint64_t jd0 = D0 + __builtin_amdgcn_readlane(g.drel, j);
-        for (uint32_t i = lane; i < jl; i += kWave) {
-            w.ring[ring_index<W::kSize>(da, jd0 + i)] = cw[jlit + i];
-        }
-    }
-    wave_order();
-}
-
 /* Executes group g of the window staged at cw into the ring at output
  * positions [dp, dp + g.end_rel), flushes whole 16-byte blocks (all of them
- * when the chunk is done) and advances dp. placed: every literal of the
- * group is in the ring already (place_literals). reserve: ring bytes past
- * the group's end that another wave may be writing meanwhile, so history
- * reads stay below them. */
+ * when the chunk is done) and advances dp. */
 template <class W>
 __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Group& g,
                                               gu8* dst, uint64_t cap, uint64_t& dp,
                                               uint64_t& flushed, uint64_t ring_lo, bool done,
-                                              uint32_t lane, bool placed = false,
-                                              uint32_t reserve = 0) {
+                                              uint32_t lane) {
     const uintptr_t da = (uintptr_t)dst;
     const uint32_t take = g.take;
     const bool full = lane < take;
@@ -688,12 +657,12 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
                    offset = g.offset, drel = g.drel;
     const uint64_t D0 = dp;
     const uint64_t D1 = dp + g.end_rel;
-    const uint64_t ring_floor = D1 + 16 + reserve > W::kSize ? D1 + 16 + reserve - W::kSize : 0;
+    const uint64_t ring_floor = D1 + 16 > W::kSize ? D1 + 16 - W::kSize : 0;
     const uint64_t hist_lo = ring_lo > ring_floor ? ring_lo : ring_floor;
     const uint64_t lit_at = D0 + drel;
     const uint64_t m_at = lit_at + lit_len;
-    /* a literal the whole wave copies in stream order (none once placed) */
-    const bool wide_lit = !placed && lit_len > kLaneLit;
+    /* a literal the whole wave copies in stream order */
+    const bool wide_lit = lit_len > kLaneLit;
     /* matches that read only history or their own literals go by their own
      * lane; the history ones load their period now, so the load overlaps
      * the literal phase */
@@ -708,7 +677,7 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
         hp = load_period(dst, y, (uint32_t)need);
     }
     /* phase 1: the lanes' literals */
-    if (!placed && active && !wide_lit) {
+    if (active && !wide_lit) {
         put_literal(w, cw, lit, lit_len, da, lit_at);
     }
     wave_order();
@@ -759,7 +728,7 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
         if (other == 0) {
             break;
         }
-        /* a wide literal (unless placed), then the match unless the member
+        /* a wide literal, then the match unless the member
          * is the terminal literal (no rest member is a lane match) */
         const uint32_t j = __builtin_ctzll(other);
         clear_bit(other, j);
@@ -767,7 +736,7 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
         const uint32_t jlit = __builtin_amdgcn_readlane(lit, j);
         const uint32_t jd = __builtin_amdgcn_readlane(drel, j);
         const uint64_t jd0 = D0 + jd;
-        if (!placed && jl > kLaneLit) {
+        if (jl > kLaneLit) {
             for (uint32_t i = lane; i < jl; i += kWave) {
                 w.ring[ring_index<W::kSize>(da, jd0 + i)] = cw[jlit + i];
             }
@@ -1504,7 +1473,6 @@ struct alignas(16) PipeRec {
     uint32_t terminal;
     uint32_t end_rel;
     uint32_t lead;              /* kGroup: the slow sequence below runs first */
-    uint32_t placed;            /* kGroup: the parser put the literals in the ring */
     uint64_t lit_src, lit_dst, lit_len_s, match_dst, offset_s, match_len_s, dp_after;
 };
 
@@ -1759,13 +1727,6 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
         /* the parser's preamble verdict (Snappy), handed over as a failed
          * window record when it is not OK */
         const int begun = parser ? chunk_begin<kFormat>(src, src_size, cap, sp, declared) : kOk;
-        /* LZ4 windows have the parser put their literals in the ring (the
-         * copier is the busier wave there); Snappy's parser is the busier
-         * one, so its copier keeps them. The record being executed
-         * meanwhile has a slow sequence: that stages through the ring, so
-         * this window's literals stay out. */
-        constexpr bool kPlace = kFormat != kFormatSnappy;
-        bool prev_slow = false;
         for (uint32_t step = 0;; step++) {
             const uint32_t b = step & 1;
             if (parser) {
@@ -1797,16 +1758,6 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         r.dp_after = pr.lead ? pr.lead_dp : dp;
                     }
                     if (pr.kind == kParseGroup) {
-                        /* the copier works below this group's start meanwhile
-                         * (and reads no history within kBatchMax of it) */
-                        const bool place = kPlace && !pr.lead && !prev_slow;
-                        if (place) {
-                            const uint8_t* cw =
-                                (b ? L.pipe.stage1 : L.pipe.w.stage) + pr.delta;
-                            place_literals(L.pipe.w, cw, pr.g, (uintptr_t)dst,
-                                           dp - pr.g.end_rel, lane);
-                        }
-                        r.placed = place ? 1u : 0u;
                         r.lit[lane] = (uint16_t)pr.g.lit;
                         r.lit_len[lane] = (uint16_t)pr.g.lit_len;
                         r.match_len[lane] = (uint16_t)pr.g.match_len;
@@ -1817,7 +1768,6 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         r.terminal = pr.g.terminal ? 1u : 0u;
                         r.end_rel = pr.g.end_rel;
                     }
-                    prev_slow = pr.kind != kParseGroup || pr.lead;
                     pdone = pr.kind == kParseFail || pr.done;
                     if (pdone) {
                         L.fin = step;
@@ -1853,8 +1803,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         const uint8_t* cw =
                             (b ? L.pipe.w.stage : L.pipe.stage1) + uniform32(r.delta);
                         execute_group(L.pipe.w, cw, g, dst, cap, dp, flushed, ring_lo,
-                                      uniform32(r.done) != 0, lane, uniform32(r.placed) != 0,
-                                      kPlace ? kBatchMax : 0);
+                                      uniform32(r.done) != 0, lane);
                     }
                 }
             }
