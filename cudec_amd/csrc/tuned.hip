@@ -776,10 +776,15 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
  * receives member j's position. The walk ends at a stop link (that position
  * is not a member), past the window, or after 64 members; it returns the
  * position after the last member and the member count. */
-template <uint32_t kC>
+template <uint32_t kC, uint32_t kMinStep>
 __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint32_t start,
                                                uint32_t& count_out, uint32_t& member) {
     static_assert(kStop >= kC * kWave + kWave, "a stop link must leave every region");
+    static_assert(kMinStep >= 2, "links go forward");
+    /* most members one region can hold, and the count at or below which
+     * the walk takes the unchecked loop there */
+    constexpr uint32_t kRegionMax = (kWave - 1) / kMinStep + 1;
+    constexpr uint32_t kFastMax = kWave - kRegionMax;
     uint32_t pos = start, count = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kC; k++) {
@@ -790,12 +795,22 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
             }
             continue; /* pos lies in a later region */
         }
-        /* Per member: read the link of pos out of lane rel, record pos as
+        /* Per member: read the link of pos out of its lane, record pos as
          * member `count`, step. The loop leaves once pos is outside this
          * region (a stop link lands far outside) or 64 members are held.
          * Only scalar registers are written apart from `member`. Two
          * members a turn, the current position alternating between pos and
-         * nx, so that no step moves it. */
+         * nx, so that no step moves it.
+         * Links only go forward, by kMinStep bytes or more (an LZ4
+         * sequence takes 3, a Snappy element 2), so a region holds at most
+         * kRegionMax members and "pos < base + 64" is the whole region
+         * check. Entered with at most kFastMax members, the walk cannot
+         * pass 64 inside the region and takes the fast loop: the position itself selects
+         * the lane (a wave64 v_readlane uses bits [5:0] of the select), and
+         * a member costs a compare, a branch and the count. Otherwise the
+         * checked loop tests the count as well. In the fast loop four
+         * instructions separate a v_readlane from the one that uses its
+         * result as the select, the 4 wait states that use needs. */
         uint32_t prev, nx, t;
         /* The count lives in m0 during the loop (saved and restored around
          * it): v_writelane takes its value and lane select from scalar
@@ -807,7 +822,22 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
         asm volatile(
             "s_mov_b32 %[m0s], m0\n\t"
             "s_mov_b32 m0, %[cnt]\n\t"
-            "s_nop 4\n"
+            "s_nop 4\n\t"
+            "s_cmp_gt_u32 %[cnt], %[fmax]\n\t"
+            "s_cbranch_scc1 walk_loop_%=\n"
+            "walk_fast_%=:\n\t"
+            "v_readlane_b32 %[nx], %[lk], %[pos]\n\t"
+            "v_writelane_b32 %[mem], %[pos], m0\n\t"
+            "s_add_u32 m0, m0, 1\n\t"
+            "s_cmp_lt_u32 %[nx], %[lim]\n\t"
+            "s_cbranch_scc0 walk_out_nx_%=\n\t"
+            "v_readlane_b32 %[pos], %[lk], %[nx]\n\t"
+            "v_writelane_b32 %[mem], %[nx], m0\n\t"
+            "s_add_u32 m0, m0, 1\n\t"
+            "s_cmp_lt_u32 %[pos], %[lim]\n\t"
+            "s_cbranch_scc1 walk_fast_%=\n\t"
+            "s_mov_b32 %[prev], %[nx]\n\t"
+            "s_branch walk_done_%=\n"
             "walk_loop_%=:\n\t"
             "v_readlane_b32 %[nx], %[lk], %[rel]\n\t"
             "v_writelane_b32 %[mem], %[pos], m0\n\t"
@@ -833,7 +863,8 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
             "s_mov_b32 m0, %[m0s]"
             : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+s"(count), [rel] "+s"(rel),
               [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [m0s] "=&s"(m0_saved)
-            : [lk] "v"(link[k]), [base] "s"(k * kWave)
+            : [lk] "v"(link[k]), [base] "s"(k * kWave), [lim] "s"(k * kWave + kWave),
+              [fmax] "s"(kFastMax)
             : "scc");
         if (pos == kStop) {
             /* prev has a stop link: it is not a member, and the walk ends there */
@@ -993,7 +1024,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
      * two-wave parser walks, leaving LDS and issue to the copier */
     const uint32_t pos = kC <= kCandSplit
                              ? walk_doubling<kC>(link, start, jump, count, member, lane)
-                             : walk_chain<kC>(link, start, count, member);
+                             : walk_chain<kC, 3>(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
@@ -1285,7 +1316,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* 
      * two-wave parser walks, leaving LDS and issue to the copier */
     const uint32_t pos = kC <= kCandSplit
                              ? walk_doubling<kC>(link, start, jump, count, member, lane)
-                             : walk_chain<kC>(link, start, count, member);
+                             : walk_chain<kC, 2>(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {

@@ -12,6 +12,11 @@ every n from 0 to 70, after literal prefixes that move the window
 boundaries: the stop lands on member 0..70, including exactly on the 64th
 and 65th position of a chain. Every stream is valid LZ4 (the oracle accepts
 it) and ends with the format's literal tail.
+
+The Snappy streams do the same with 2-byte copy elements (32 to a 64-byte
+region of a window, where an LZ4 sequence takes at least 3 bytes) in front
+of a literal too long for a window member, so a chain passes 32 and 64
+members inside one region.
 """
 import numpy as np
 import pytest
@@ -63,6 +68,55 @@ def chains():
     return streams, caps, names
 
 
+def snappy_walk_stream(seed: int, prefix: int) -> bytes:
+    """A `prefix`-byte literal, then for n = 0..70: n copy-1 elements of
+    4..11 bytes (2 bytes each) and one 300-byte literal."""
+    rng = np.random.default_rng(seed)
+    body = bytearray()
+    pos = 0
+
+    def literal(data: bytes):
+        nonlocal pos
+        n = len(data) - 1
+        if n < 60:
+            body.append(n << 2)
+        elif n < 256:
+            body.extend((60 << 2, n))
+        else:
+            body.extend((61 << 2, n & 0xFF, n >> 8))
+        body.extend(data)
+        pos += len(data)
+
+    literal(rng.integers(0, 256, prefix, dtype=np.uint8).tobytes())
+    for n in range(71):
+        for _ in range(n):
+            length = int(rng.integers(4, 12))
+            off = int(rng.integers(1, min(pos, 2047) + 1))
+            body += bytes([((length - 4) << 2) | ((off >> 8) << 5) | 1, off & 0xFF])
+            pos += length
+        literal(rng.integers(0, 256, 300, dtype=np.uint8).tobytes())
+    head = bytearray()
+    v = pos
+    while v >= 0x80:
+        head.append((v & 0x7F) | 0x80)
+        v >>= 7
+    head.append(v)
+    return bytes(head + body)
+
+
+@pytest.fixture(scope="module")
+def snappy_chains():
+    streams, caps, names = [], [], []
+    for k, prefix in enumerate((1, 2, 7, 64, 100, 1000)):
+        s = snappy_walk_stream(0x5A17 + 13 * k, prefix)
+        st, out = oracle_decode("snappy", s, 1 << 20)
+        assert st == libs.OK, f"generator wrote an invalid stream (prefix {prefix})"
+        streams.append(s)
+        caps.append(len(out))
+        names.append(f"snappy-prefix{prefix}")
+    return streams, caps, names
+
+
 @pytest.fixture()
 def restore_threshold():
     st, old = cudec_amd.lz4_pipeline_threshold()
@@ -77,3 +131,11 @@ def test_chain_edges_exact(chains, restore_threshold, threshold):
     assert cudec_amd.set_lz4_pipeline_threshold(threshold) == 0
     print(f"[walk edges] {len(streams)} chunks, threshold {threshold}", flush=True)
     check_against_oracle("lz4", "tuned", streams, caps, names)
+
+
+@pytest.mark.parametrize("threshold", [0, (1 << 64) - 1])
+def test_snappy_chain_edges_exact(snappy_chains, restore_threshold, threshold):
+    streams, caps, names = snappy_chains
+    assert cudec_amd.set_lz4_pipeline_threshold(threshold) == 0
+    print(f"[walk edges] snappy {len(streams)} chunks, threshold {threshold}", flush=True)
+    check_against_oracle("snappy", "tuned", streams, caps, names)
