@@ -789,10 +789,7 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
 #pragma unroll
     for (uint32_t k = 0; k < kC; k++) {
         uint32_t rel = pos - k * kWave;
-        if ((rel | count) >= kWave) {
-            if (count == kWave) {
-                break;
-            }
+        if (rel >= kWave) {
             continue; /* pos lies in a later region */
         }
         /* Per member: read the link of pos out of its lane, record pos as
@@ -811,7 +808,7 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
          * checked loop tests the count as well. In the fast loop four
          * instructions separate a v_readlane from the one that uses its
          * result as the select, the 4 wait states that use needs. */
-        uint32_t prev, nx, t;
+        uint32_t prev, nx, t, fin;
         /* The count lives in m0 during the loop (saved and restored around
          * it): v_writelane takes its value and lane select from scalar
          * registers, and only m0 as the lane select keeps it within one
@@ -860,20 +857,26 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
             "s_mov_b32 %[pos], %[nx]\n"
             "walk_done_%=:\n\t"
             "s_mov_b32 %[cnt], m0\n\t"
-            "s_mov_b32 m0, %[m0s]"
+            "s_mov_b32 m0, %[m0s]\n\t"
+            /* a stop link: prev is no member, and the walk ends there */
+            "s_cmpk_eq_u32 %[pos], 0xffff\n\t"
+            "s_cbranch_scc0 walk_nostop_%=\n\t"
+            "s_mov_b32 %[pos], %[prev]\n\t"
+            "s_sub_u32 %[cnt], %[cnt], 1\n\t"
+            "s_mov_b32 %[fin], 1\n\t"
+            "s_branch walk_end_%=\n"
+            "walk_nostop_%=:\n\t"
+            "s_cmp_eq_u32 %[cnt], 64\n\t"
+            "s_cselect_b32 %[fin], 1, 0\n"
+            "walk_end_%=:"
             : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+s"(count), [rel] "+s"(rel),
-              [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [m0s] "=&s"(m0_saved)
+              [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [m0s] "=&s"(m0_saved),
+              [fin] "=&s"(fin)
             : [lk] "v"(link[k]), [base] "s"(k * kWave), [lim] "s"(k * kWave + kWave),
               [fmax] "s"(kFastMax)
             : "scc");
-        if (pos == kStop) {
-            /* prev has a stop link: it is not a member, and the walk ends there */
-            pos = prev;
-            count--;
-            break;
-        }
-        if (count == kWave) {
-            break;
+        if (fin != 0) {
+            break; /* a stop link, or 64 members */
         }
     }
     count_out = uniform32(count);
