@@ -158,6 +158,19 @@ __device__ __forceinline__ uint32_t stage_into(uint8_t* stage, gcu8* src,
     return (uint32_t)(a - base);
 }
 
+/* Loads one dword per lane of the 1 KiB after the window staged at sp
+ * (16 bytes apart, every one inside the source), so that the next window's
+ * stage finds those bytes in L2. The caller holds the value until the
+ * window is parsed, which keeps the loads off its critical path. */
+template <uint32_t kC>
+__device__ __forceinline__ uint32_t touch_ahead(gcu8* src, uint64_t src_size, uint64_t sp,
+                                                uint32_t lane) {
+    const uintptr_t base = (uintptr_t)(src + sp) & ~(uintptr_t)15;
+    const uintptr_t g = base + 16u * (stage_bytes(kC) / 16 + 1) + 16u * lane;
+    const uintptr_t end = (uintptr_t)src + src_size;
+    return g + 4 <= end ? *(gcu32*)g : 0u;
+}
+
 /* ------------------------------------------------------ candidate parse */
 
 struct Cand {
@@ -975,7 +988,8 @@ struct Parsed {
 template <uint32_t kC>
 __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, gcu8* src,
                                                uint64_t src_size, uint64_t cap, uint64_t& sp,
-                                               uint64_t& dp, uint32_t lane) {
+                                               uint64_t& dp, uint32_t lane,
+                                               uint32_t& touch) {
     Parsed out;
     out.kind = kParseFail;
     out.status = kOk;
@@ -984,6 +998,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
     out.lead_dp = 0;
     const uint64_t base = sp; /* window start; member positions are relative to it */
     out.delta = stage_into<kC>(stage, src, src_size, sp, lane);
+    touch = touch_ahead<kC>(src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
 
     /* speculate: the link of each candidate position */
@@ -1262,7 +1277,8 @@ template <uint32_t kC>
 __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* jump, gcu8* src,
                                                       uint64_t src_size, uint64_t cap,
                                                       uint64_t declared, uint64_t& sp,
-                                                      uint64_t& dp, uint32_t lane) {
+                                                      uint64_t& dp, uint32_t lane,
+                                                      uint32_t& touch) {
     Parsed out;
     out.kind = kParseFail;
     out.status = kOk;
@@ -1271,6 +1287,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* 
     out.lead_dp = 0;
     const uint64_t base = sp;
     out.delta = stage_into<kC>(stage, src, src_size, sp, lane);
+    touch = touch_ahead<kC>(src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
     /* source bytes from the window start, clamped: a fused copy must start
      * before the source end */
@@ -1447,11 +1464,13 @@ template <int kFormat, uint32_t kC>
 __device__ __forceinline__ Parsed parse_any(uint8_t* stage, uint16_t* jump, gcu8* src,
                                             uint64_t src_size,
                                             uint64_t cap, uint64_t declared, uint64_t& sp,
-                                            uint64_t& dp, uint32_t lane) {
+                                            uint64_t& dp, uint32_t lane,
+                                            uint32_t& touch) {
     if (kFormat == kFormatSnappy) {
-        return parse_window_snappy<kC>(stage, jump, src, src_size, cap, declared, sp, dp, lane);
+        return parse_window_snappy<kC>(stage, jump, src, src_size, cap, declared, sp, dp, lane,
+                                       touch);
     }
-    return parse_window<kC>(stage, jump, src, src_size, cap, sp, dp, lane);
+    return parse_window<kC>(stage, jump, src, src_size, cap, sp, dp, lane, touch);
 }
 
 /* Start of a chunk: the Snappy preamble and its declared length against the
@@ -1549,9 +1568,10 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
             status = kCorrupt;
             break;
         }
+        uint32_t touch = 0;
         const Parsed pr =
             parse_any<kFormat, kCandSplit>(w.stage, w.jump, src, src_size, cap, declared, sp, pdp,
-                                           lane);
+                                           lane, touch);
         if (pr.kind == kParseFail) {
             status = pr.status;
             break;
@@ -1565,6 +1585,8 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
         } else {
             execute_slow(w, src, dst, cap, pr.q, pdp, dp, flushed, ring_lo, lane);
         }
+        /* the look-ahead loads land by now */
+        asm volatile("" ::"v"(touch));
         if (pr.done) {
             break;
         }
@@ -1767,6 +1789,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                 if (!pdone) {
                     PipeRec& r = L.pipe.rec[b];
                     Parsed pr;
+                    uint32_t touch = 0;
                     if (begun != kOk || fuel-- == 0) {
                         pr.kind = kParseFail;
                         pr.status = begun != kOk ? begun : kCorrupt;
@@ -1776,7 +1799,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         constexpr uint32_t kC =
                             kFormat == kFormatSnappy ? kCandPipeSnappy : kCandPipe;
                         pr = parse_any<kFormat, kC>(b ? L.pipe.stage1 : L.pipe.w.stage, nullptr,
-                                                    src, src_size, cap, declared, sp, dp, lane);
+                                                    src, src_size, cap, declared, sp, dp, lane,
+                                                    touch);
                     }
                     r.kind = pr.kind;
                     r.status = pr.status;
@@ -1806,6 +1830,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                     if (pdone) {
                         L.fin = step;
                     }
+                    /* the look-ahead loads land by now */
+                    asm volatile("" ::"v"(touch));
                 }
             } else if (step > 0 && status == kOk) {
                 const PipeRec& r = L.pipe.rec[b ^ 1];
