@@ -49,16 +49,18 @@ namespace cudec_amd {
 namespace {
 
 /* Candidates per lane: a window spans kC * 64 candidate positions. The
- * two-wave form parses wider windows than the one-wave form: its parser
- * runs beside the copier, and a wider window fills more lanes of every
- * lane-parallel step, while a lone wave pays the wider speculation on its
- * own critical path. Snappy's two-wave windows are 768 candidates: with
- * them its two-wave path fits 96 VGPRs, so Snappy launches run five waves
- * per SIMD throughout (1024-candidate LZ4 windows at four waves per SIMD
- * are as fast as 768 at five). */
+ * LZ4 two-wave form parses wider windows than the one-wave form: its
+ * parser runs beside the copier, and a wider window fills more lanes of
+ * every lane-parallel step, while a lone wave pays the wider speculation
+ * on its own critical path. Snappy's two-wave windows are 448 candidates:
+ * a window holds at most 64 members, and Snappy elements (2 to 5 bytes of
+ * the source, mostly) fill those within a few hundred bytes, so wider
+ * windows only add speculation to the busier parser wave (768: 35.2 ms,
+ * 576: 32.1, 448: 30.7, 384: 31.9 on the uneven batch); its two-wave path
+ * fits 96 VGPRs, so Snappy launches run five waves per SIMD throughout. */
 constexpr uint32_t kCandSplit = 8;
 constexpr uint32_t kCandPipe = 16;
-constexpr uint32_t kCandPipeSnappy = 12;
+constexpr uint32_t kCandPipeSnappy = 7;
 /* staged bytes a window of kC candidates per lane reads (the window plus
  * the longest candidate that starts in it) */
 constexpr uint32_t stage_bytes(uint32_t kc) { return kc * 64u + 512u; }
@@ -984,8 +986,11 @@ struct Parsed {
 
 /* Stages the window at sp into `stage`, speculates, walks and resolves it
  * (src/lz4_block.h:95-258 for every member, in parallel) and advances
- * sp and dp past what it decided. Reads the source only. */
-template <uint32_t kC>
+ * sp and dp past what it decided. Reads the source only. kDoubling: walk
+ * the chain by pointer doubling over `jump` (the one-wave form, where the
+ * wave's own latency is the pace), else serially (the two-wave parser,
+ * which leaves LDS and issue to the copier; `jump` is unused). */
+template <uint32_t kC, bool kDoubling>
 __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, gcu8* src,
                                                uint64_t src_size, uint64_t cap, uint64_t& sp,
                                                uint64_t& dp, uint32_t lane,
@@ -1038,9 +1043,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
     /* walk the true chain */
     uint32_t count = 0;
     uint32_t member = 0; /* lane j: position of member j */
-    /* a one-wave window doubles (the wave's own latency is the pace); the
-     * two-wave parser walks, leaving LDS and issue to the copier */
-    const uint32_t pos = kC <= kCandSplit
+    const uint32_t pos = kDoubling
                              ? walk_doubling<kC>(link, start, jump, count, member, lane)
                              : walk_chain<kC, 3>(link, start, count, member);
 
@@ -1272,8 +1275,8 @@ __device__ __forceinline__ Cand parse_candidate_snappy(const uint8_t* cw, uint32
  * rungs run in parallel (every reject is CORRUPT_INPUT, so the first
  * failing member in stream order decides), and the member whose position
  * is exactly the end of the source ends the chunk when the output so far
- * equals the declared length. */
-template <uint32_t kC>
+ * equals the declared length. kDoubling as for parse_window. */
+template <uint32_t kC, bool kDoubling>
 __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* jump, gcu8* src,
                                                       uint64_t src_size, uint64_t cap,
                                                       uint64_t declared, uint64_t& sp,
@@ -1332,9 +1335,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* 
     }
     uint32_t count = 0;
     uint32_t member = 0;
-    /* a one-wave window doubles (the wave's own latency is the pace); the
-     * two-wave parser walks, leaving LDS and issue to the copier */
-    const uint32_t pos = kC <= kCandSplit
+    const uint32_t pos = kDoubling
                              ? walk_doubling<kC>(link, start, jump, count, member, lane)
                              : walk_chain<kC, 2>(link, start, count, member);
 
@@ -1460,17 +1461,17 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* 
 }
 
 /* A window of either format. */
-template <int kFormat, uint32_t kC>
+template <int kFormat, uint32_t kC, bool kDoubling>
 __device__ __forceinline__ Parsed parse_any(uint8_t* stage, uint16_t* jump, gcu8* src,
                                             uint64_t src_size,
                                             uint64_t cap, uint64_t declared, uint64_t& sp,
                                             uint64_t& dp, uint32_t lane,
                                             uint32_t& touch) {
     if (kFormat == kFormatSnappy) {
-        return parse_window_snappy<kC>(stage, jump, src, src_size, cap, declared, sp, dp, lane,
-                                       touch);
+        return parse_window_snappy<kC, kDoubling>(stage, jump, src, src_size, cap, declared, sp,
+                                                  dp, lane, touch);
     }
-    return parse_window<kC>(stage, jump, src, src_size, cap, sp, dp, lane, touch);
+    return parse_window<kC, kDoubling>(stage, jump, src, src_size, cap, sp, dp, lane, touch);
 }
 
 /* Start of a chunk: the Snappy preamble and its declared length against the
@@ -1570,8 +1571,8 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
         }
         uint32_t touch = 0;
         const Parsed pr =
-            parse_any<kFormat, kCandSplit>(w.stage, w.jump, src, src_size, cap, declared, sp, pdp,
-                                           lane, touch);
+            parse_any<kFormat, kCandSplit, true>(w.stage, w.jump, src, src_size, cap, declared,
+                                                 sp, pdp, lane, touch);
         if (pr.kind == kParseFail) {
             status = pr.status;
             break;
@@ -1798,9 +1799,9 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                     } else {
                         constexpr uint32_t kC =
                             kFormat == kFormatSnappy ? kCandPipeSnappy : kCandPipe;
-                        pr = parse_any<kFormat, kC>(b ? L.pipe.stage1 : L.pipe.w.stage, nullptr,
-                                                    src, src_size, cap, declared, sp, dp, lane,
-                                                    touch);
+                        pr = parse_any<kFormat, kC, false>(
+                            b ? L.pipe.stage1 : L.pipe.w.stage, nullptr, src, src_size, cap,
+                            declared, sp, dp, lane, touch);
                     }
                     r.kind = pr.kind;
                     r.status = pr.status;
