@@ -791,7 +791,7 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
  * receives member j's position. The walk ends at a stop link (that position
  * is not a member), past the window, or after 64 members; it returns the
  * position after the last member and the member count. */
-template <uint32_t kC, uint32_t kMinStep>
+template <uint32_t kC, uint32_t kMinStep, bool kStraight>
 __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint32_t start,
                                                uint32_t& count_out, uint32_t& member) {
     static_assert(kStop >= kC * kWave + kWave, "a stop link must leave every region");
@@ -800,12 +800,16 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
      * the walk takes the unchecked loop there */
     constexpr uint32_t kRegionMax = (kWave - 1) / kMinStep + 1;
     constexpr uint32_t kFastMax = kWave - kRegionMax;
-    uint32_t pos = start, count = 0;
+    uint32_t pos = start, count = 0, fin = 0;
+    /* kStraight: a straight run of guarded regions with no early exit (an
+     * exit edge out of the unrolled loop has the compiler re-materialise
+     * live flags at every region; Snappy's walk gains 2% without it, LZ4's
+     * loses 0.4%) */
 #pragma unroll
     for (uint32_t k = 0; k < kC; k++) {
         uint32_t rel = pos - k * kWave;
-        if (rel >= kWave) {
-            continue; /* pos lies in a later region */
+        if ((kStraight && fin != 0) || rel >= kWave) {
+            continue; /* the walk has ended, or pos lies in a later region */
         }
         /* Per member: read the link of pos out of its lane, record pos as
          * member `count`, step. The loop leaves once pos is outside this
@@ -823,7 +827,7 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
          * checked loop tests the count as well. In the fast loop four
          * instructions separate a v_readlane from the one that uses its
          * result as the select, the 4 wait states that use needs. */
-        uint32_t prev, nx, t, fin;
+        uint32_t prev, nx, t;
         /* The count lives in m0 during the loop (saved and restored around
          * it): v_writelane takes its value and lane select from scalar
          * registers, and only m0 as the lane select keeps it within one
@@ -890,7 +894,7 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
             : [lk] "v"(link[k]), [base] "s"(k * kWave), [lim] "s"(k * kWave + kWave),
               [fmax] "s"(kFastMax)
             : "scc");
-        if (fin != 0) {
+        if (!kStraight && fin != 0) {
             break; /* a stop link, or 64 members */
         }
     }
@@ -1045,7 +1049,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
     uint32_t member = 0; /* lane j: position of member j */
     const uint32_t pos = kDoubling
                              ? walk_doubling<kC>(link, start, jump, count, member, lane)
-                             : walk_chain<kC, 3>(link, start, count, member);
+                             : walk_chain<kC, 3, false>(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
@@ -1337,7 +1341,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* 
     uint32_t member = 0;
     const uint32_t pos = kDoubling
                              ? walk_doubling<kC>(link, start, jump, count, member, lane)
-                             : walk_chain<kC, 2>(link, start, count, member);
+                             : walk_chain<kC, 2, true>(link, start, count, member);
 
     bool slow = count == 0;
     if (!slow) {
