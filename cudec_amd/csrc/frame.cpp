@@ -103,23 +103,23 @@ cudec_status decode_blocks(const unsigned char* f, const std::vector<Block>& blo
     }
     const size_t n = ci.size();
     std::vector<size_t> decoded(blocks.size(), 0);
-    StreamOwner stream;
-    DeviceBuffer d_src, d_dst, d_meta, d_res;
+    Stream stream;
+    DeviceRegion d_src, d_dst, d_meta, d_res;
     if (n != 0) {
         if (n > SIZE_MAX / block_max || n > SIZE_MAX / (4 * sizeof(void*))) {
             return CUDEC_ERR_CORRUPT_INPUT;
         }
-        if (stream.create() != hipSuccess || d_src.ensure(total_src) != hipSuccess ||
-            d_dst.ensure(n * block_max) != hipSuccess ||
-            d_meta.ensure(4 * n * sizeof(void*)) != hipSuccess ||
-            d_res.ensure(n * sizeof(cudec_chunk_result)) != hipSuccess) {
+        if (make_stream(stream) != hipSuccess || d_src.reserve(total_src) != hipSuccess ||
+            d_dst.reserve(n * block_max) != hipSuccess ||
+            d_meta.reserve(4 * n * sizeof(void*)) != hipSuccess ||
+            d_res.reserve(n * sizeof(cudec_chunk_result)) != hipSuccess) {
             return CUDEC_ERR_CUDA;
         }
         std::vector<unsigned char> stage(total_src);
         const size_t stride = n * sizeof(void*);
         std::vector<unsigned char> meta(4 * stride);
-        unsigned char* ds = static_cast<unsigned char*>(d_src.p);
-        unsigned char* dd = static_cast<unsigned char*>(d_dst.p);
+        unsigned char* ds = static_cast<unsigned char*>(d_src.data());
+        unsigned char* dd = static_cast<unsigned char*>(d_dst.data());
         size_t so = 0;
         for (size_t k = 0; k < n; k++) {
             const Block& b = blocks[ci[k]];
@@ -132,22 +132,22 @@ cudec_status decode_blocks(const unsigned char* f, const std::vector<Block>& blo
             std::memcpy(meta.data() + 3 * stride + k * sizeof(size_t), &block_max, sizeof(size_t));
             so += b.len;
         }
-        hipStream_t st = stream.s;
+        hipStream_t st = stream.get();
         if (hipMemcpyAsync(ds, stage.data(), total_src, hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(d_meta.p, meta.data(), meta.size(), hipMemcpyHostToDevice, st) !=
+            hipMemcpyAsync(d_meta.data(), meta.data(), meta.size(), hipMemcpyHostToDevice, st) !=
                 hipSuccess) {
             (void)hipStreamSynchronize(st);
             return CUDEC_ERR_CUDA;
         }
-        unsigned char* dm = static_cast<unsigned char*>(d_meta.p);
+        unsigned char* dm = static_cast<unsigned char*>(d_meta.data());
         const cudec_status launched = cudec_lz4_decompress_batch(
             reinterpret_cast<const void* const*>(dm), reinterpret_cast<const size_t*>(dm + stride),
             reinterpret_cast<void* const*>(dm + 2 * stride),
             reinterpret_cast<const size_t*>(dm + 3 * stride), n,
-            static_cast<cudec_chunk_result*>(d_res.p), reinterpret_cast<cudec_stream_t>(st));
+            static_cast<cudec_chunk_result*>(d_res.data()), reinterpret_cast<cudec_stream_t>(st));
         std::vector<cudec_chunk_result> res(n);
         const bool copied = launched == CUDEC_OK &&
-                            hipMemcpyAsync(res.data(), d_res.p, n * sizeof(cudec_chunk_result),
+                            hipMemcpyAsync(res.data(), d_res.data(), n * sizeof(cudec_chunk_result),
                                            hipMemcpyDeviceToHost, st) == hipSuccess;
         /* the staging vectors must outlive the asynchronous uploads */
         if (hipStreamSynchronize(st) != hipSuccess) {
@@ -192,8 +192,8 @@ cudec_status decode_blocks(const unsigned char* f, const std::vector<Block>& blo
             run_len += len;
         } else {
             if (run_len != 0) {
-                err = hipMemcpyAsync(out + run_out, static_cast<unsigned char*>(d_dst.p) + run_dev,
-                                     run_len, hipMemcpyDeviceToHost, stream.s);
+                err = hipMemcpyAsync(out + run_out, static_cast<unsigned char*>(d_dst.data()) + run_dev,
+                                     run_len, hipMemcpyDeviceToHost, stream.get());
             }
             run_dev = dev;
             run_out = pos;
@@ -203,10 +203,10 @@ cudec_status decode_blocks(const unsigned char* f, const std::vector<Block>& blo
         k++;
     }
     if (err == hipSuccess && run_len != 0) {
-        err = hipMemcpyAsync(out + run_out, static_cast<unsigned char*>(d_dst.p) + run_dev, run_len,
-                             hipMemcpyDeviceToHost, stream.s);
+        err = hipMemcpyAsync(out + run_out, static_cast<unsigned char*>(d_dst.data()) + run_dev, run_len,
+                             hipMemcpyDeviceToHost, stream.get());
     }
-    if (n != 0 && hipStreamSynchronize(stream.s) != hipSuccess) {
+    if (n != 0 && hipStreamSynchronize(stream.get()) != hipSuccess) {
         err = hipErrorUnknown;
     }
     if (err != hipSuccess) {

@@ -34,14 +34,14 @@ constexpr size_t kWaveBudget = size_t{1} << 30;
 constexpr size_t kWaveChunksMax = 16384;
 
 struct Slot {
-    PinnedBuffer p_src;
-    DeviceBuffer d_src;
-    PinnedBuffer p_meta;
-    DeviceBuffer d_meta;
-    DeviceBuffer d_dst;
-    PinnedBuffer p_dst;
-    EventOwner staged;  /* the H2D of this slot's pinned source has completed */
-    EventOwner landed;  /* this slot's results and output are in pinned memory */
+    PinnedRegion p_src;
+    DeviceRegion d_src;
+    PinnedRegion p_meta;
+    DeviceRegion d_meta;
+    DeviceRegion d_dst;
+    PinnedRegion p_dst;
+    Event staged;  /* the H2D of this slot's pinned source has completed */
+    Event landed;  /* this slot's results and output are in pinned memory */
     bool staged_pending = false;
     bool landed_pending = false;
     size_t begin = 0, count = 0;
@@ -50,10 +50,10 @@ struct Slot {
 }  // namespace cudec_amd
 
 struct cudec_stream_ctx {
-    cudec_amd::StreamOwner stream;
+    cudec_amd::Stream stream;
     cudec_amd::Slot slot[2];
-    cudec_amd::DeviceBuffer d_results;
-    cudec_amd::PinnedBuffer p_results;
+    cudec_amd::DeviceRegion d_results;
+    cudec_amd::PinnedRegion p_results;
     bool poisoned = false;
 };
 
@@ -113,7 +113,7 @@ size_t choose_wave(const size_t* src_sizes, const size_t* caps, size_t n, bool h
  * capacity is a device fault, never a copy length (src/stream.cpp:163-184). */
 cudec_status unpack_wave(const Slot& s, cudec_chunk_result* pres, void* const* dst_ptrs,
                          const size_t* caps) {
-    const unsigned char* base = static_cast<const unsigned char*>(s.p_dst.p);
+    const unsigned char* base = static_cast<const unsigned char*>(s.p_dst.data());
     size_t off = 0;
     for (size_t j = 0; j < s.count; j++) {
         const size_t i = s.begin + j;
@@ -169,23 +169,23 @@ cudec_status decode_stream(cudec_stream_ctx& ctx, const void* const* h_src_ptrs,
     bool ok = true;
     for (int k = 0; k < 2 && ok; k++) {
         Slot& s = ctx.slot[k];
-        ok = s.p_src.ensure(max_src) == hipSuccess && s.d_src.ensure(max_src) == hipSuccess &&
-             s.p_meta.ensure(meta_bytes) == hipSuccess &&
-             s.d_meta.ensure(meta_bytes) == hipSuccess &&
-             (!host_out || (s.d_dst.ensure(max_dst) == hipSuccess &&
-                            s.p_dst.ensure(max_dst) == hipSuccess));
+        ok = s.p_src.reserve(max_src) == hipSuccess && s.d_src.reserve(max_src) == hipSuccess &&
+             s.p_meta.reserve(meta_bytes) == hipSuccess &&
+             s.d_meta.reserve(meta_bytes) == hipSuccess &&
+             (!host_out || (s.d_dst.reserve(max_dst) == hipSuccess &&
+                            s.p_dst.reserve(max_dst) == hipSuccess));
         s.staged_pending = false;
         s.landed_pending = false;
     }
-    ok = ok && ctx.d_results.ensure(res_bytes) == hipSuccess &&
-         ctx.p_results.ensure(res_bytes) == hipSuccess;
+    ok = ok && ctx.d_results.reserve(res_bytes) == hipSuccess &&
+         ctx.p_results.reserve(res_bytes) == hipSuccess;
     if (!ok) {
         ctx.poisoned = true;
         return CUDEC_ERR_CUDA;
     }
-    cudec_chunk_result* pres = static_cast<cudec_chunk_result*>(ctx.p_results.p);
+    cudec_chunk_result* pres = static_cast<cudec_chunk_result*>(ctx.p_results.data());
     stamp(pres, n, CUDEC_ERR_CUDA);
-    hipStream_t st = ctx.stream.s;
+    hipStream_t st = ctx.stream.get();
     cudec_status failure = CUDEC_OK;
 
     for (size_t w = 0; w < waves && failure == CUDEC_OK; w++) {
@@ -193,14 +193,14 @@ cudec_status decode_stream(cudec_stream_ctx& ctx, const void* const* h_src_ptrs,
         /* the slot's previous wave: its pinned source must be uploaded, and
          * on the host-output path its output unpacked, before reuse */
         if (s.staged_pending) {
-            if (hipEventSynchronize(s.staged.e) != hipSuccess) {
+            if (hipEventSynchronize(s.staged.get()) != hipSuccess) {
                 failure = CUDEC_ERR_CUDA;
                 break;
             }
             s.staged_pending = false;
         }
         if (s.landed_pending) {
-            if (hipEventSynchronize(s.landed.e) != hipSuccess) {
+            if (hipEventSynchronize(s.landed.get()) != hipSuccess) {
                 failure = CUDEC_ERR_CUDA;
                 break;
             }
@@ -214,10 +214,10 @@ cudec_status decode_stream(cudec_stream_ctx& ctx, const void* const* h_src_ptrs,
         }
         s.begin = w * wave;
         s.count = (s.begin + wave < n ? s.begin + wave : n) - s.begin;
-        unsigned char* ps = static_cast<unsigned char*>(s.p_src.p);
-        unsigned char* ds = static_cast<unsigned char*>(s.d_src.p);
-        unsigned char* dd = static_cast<unsigned char*>(s.d_dst.p);
-        unsigned char* pm = static_cast<unsigned char*>(s.p_meta.p);
+        unsigned char* ps = static_cast<unsigned char*>(s.p_src.data());
+        unsigned char* ds = static_cast<unsigned char*>(s.d_src.data());
+        unsigned char* dd = static_cast<unsigned char*>(s.d_dst.data());
+        unsigned char* pm = static_cast<unsigned char*>(s.p_meta.data());
         const void** m_src = reinterpret_cast<const void**>(pm);
         size_t* m_size = reinterpret_cast<size_t*>(pm + meta_stride);
         void** m_dst = reinterpret_cast<void**>(pm + 2 * meta_stride);
@@ -239,11 +239,11 @@ cudec_status decode_stream(cudec_stream_ctx& ctx, const void* const* h_src_ptrs,
             }
             so += h_src_sizes[i];
         }
-        unsigned char* dm = static_cast<unsigned char*>(s.d_meta.p);
-        cudec_chunk_result* dres = static_cast<cudec_chunk_result*>(ctx.d_results.p) + s.begin;
+        unsigned char* dm = static_cast<unsigned char*>(s.d_meta.data());
+        cudec_chunk_result* dres = static_cast<cudec_chunk_result*>(ctx.d_results.data()) + s.begin;
         if ((so != 0 && hipMemcpyAsync(ds, ps, so, hipMemcpyHostToDevice, st) != hipSuccess) ||
             hipMemcpyAsync(dm, pm, meta_bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipEventRecord(s.staged.e, st) != hipSuccess) {
+            hipEventRecord(s.staged.get(), st) != hipSuccess) {
             failure = CUDEC_ERR_CUDA;
             break;
         }
@@ -261,8 +261,8 @@ cudec_status decode_stream(cudec_stream_ctx& ctx, const void* const* h_src_ptrs,
         if (hipMemcpyAsync(pres + s.begin, dres, s.count * sizeof(cudec_chunk_result),
                            hipMemcpyDeviceToHost, st) != hipSuccess ||
             (host_out && dof != 0 &&
-             hipMemcpyAsync(s.p_dst.p, dd, dof, hipMemcpyDeviceToHost, st) != hipSuccess) ||
-            hipEventRecord(s.landed.e, st) != hipSuccess) {
+             hipMemcpyAsync(s.p_dst.data(), dd, dof, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+            hipEventRecord(s.landed.get(), st) != hipSuccess) {
             failure = CUDEC_ERR_CUDA;
             break;
         }
@@ -321,10 +321,11 @@ cudec_status cudec_stream_ctx_create(cudec_stream_ctx** out_ctx) {
     if (ctx == nullptr) {
         return CUDEC_ERR_CUDA;
     }
-    if (ctx->stream.create() != hipSuccess || ctx->slot[0].staged.create() != hipSuccess ||
-        ctx->slot[0].landed.create() != hipSuccess ||
-        ctx->slot[1].staged.create() != hipSuccess ||
-        ctx->slot[1].landed.create() != hipSuccess) {
+    if (make_stream(ctx->stream) != hipSuccess ||
+        make_sync_event(ctx->slot[0].staged) != hipSuccess ||
+        make_sync_event(ctx->slot[0].landed) != hipSuccess ||
+        make_sync_event(ctx->slot[1].staged) != hipSuccess ||
+        make_sync_event(ctx->slot[1].landed) != hipSuccess) {
         delete ctx;
         return CUDEC_ERR_CUDA;
     }
@@ -363,8 +364,8 @@ void cudec_stream_ctx_destroy(cudec_stream_ctx* ctx) {
     if (ctx == nullptr) {
         return;
     }
-    if (ctx->stream.s != nullptr) {
-        (void)hipStreamSynchronize(ctx->stream.s);
+    if (ctx->stream.get() != nullptr) {
+        (void)hipStreamSynchronize(ctx->stream.get());
     }
     delete ctx;
 }
