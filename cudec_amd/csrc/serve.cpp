@@ -39,6 +39,12 @@ public:
         }
     }
     bool take(size_t n, size_t* off) {
+        if (n == 0) {
+            /* a request of zero-capacity chunks holds no pool bytes: it is
+             * always admissible, even with a zero budget or a full pool */
+            *off = 0;
+            return true;
+        }
         for (auto it = free_.begin(); it != free_.end(); ++it) {
             if (it->second >= n) {
                 const size_t o = it->first, rest = it->second - n;
@@ -80,14 +86,14 @@ private:
 constexpr size_t kTableBytes = 4 * sizeof(uint64_t);
 
 struct Slot {
-    StreamOwner stream;
-    EventOwner done;
-    PinnedBuffer h_tables;
-    DeviceBuffer d_tables;
-    PinnedBuffer h_results;
-    DeviceBuffer d_results;
-    PinnedBuffer h_digests;
-    DeviceBuffer d_digests;
+    Stream stream;
+    Event done;
+    PinnedRegion h_tables;
+    DeviceRegion d_tables;
+    PinnedRegion h_results;
+    DeviceRegion d_results;
+    PinnedRegion h_digests;
+    DeviceRegion d_digests;
     std::vector<size_t> requests;
     bool busy = false;
 };
@@ -107,14 +113,14 @@ public:
           pool_(budget), state_(nreq) {}
 
     cudec_status run(cudec_mi355x_serve_stats* stats) {
-        if (pool_mem_.ensure(budget_) != hipSuccess) {
+        if (pool_mem_.reserve(budget_) != hipSuccess) {
             return CUDEC_ERR_CUDA;
         }
         slots_.reserve((size_t)inflight_);
         for (int i = 0; i < inflight_; i++) {
             slots_.push_back(std::make_unique<Slot>());
-            if (slots_.back()->stream.create() != hipSuccess ||
-                slots_.back()->done.create() != hipSuccess) {
+            if (make_stream(slots_.back()->stream) != hipSuccess ||
+                make_sync_event(slots_.back()->done) != hipSuccess) {
                 return CUDEC_ERR_CUDA;
             }
         }
@@ -144,7 +150,7 @@ public:
                 if (!s->busy) {
                     continue;
                 }
-                const hipError_t q = hipEventQuery(s->done.e);
+                const hipError_t q = hipEventQuery(s->done.get());
                 if (q == hipErrorNotReady) {
                     continue;
                 }
@@ -205,7 +211,7 @@ private:
     cudec_status fail() {
         for (auto& s : slots_) {
             if (s->busy) {
-                (void)hipStreamSynchronize(s->stream.s);
+                (void)hipStreamSynchronize(s->stream.get());
             }
         }
         return CUDEC_ERR_CUDA;
@@ -217,32 +223,32 @@ private:
             n += reqs_[r].chunk_count;
         }
         const size_t want = n * kTableBytes;
-        if (s.h_tables.size < want) {
+        if (s.h_tables.capacity() < want) {
             /* grow by doubling so a growing load settles after a few launches */
-            const size_t grow = std::max(want, 2 * s.h_tables.size);
-            if (s.h_tables.ensure(grow) != hipSuccess || s.d_tables.ensure(grow) != hipSuccess) {
+            const size_t grow = std::max(want, 2 * s.h_tables.capacity());
+            if (s.h_tables.reserve(grow) != hipSuccess || s.d_tables.reserve(grow) != hipSuccess) {
                 return false;
             }
         }
         const size_t rb = n * sizeof(cudec_chunk_result);
-        if (s.h_results.size < rb) {
-            const size_t grow = std::max(rb, 2 * s.h_results.size);
-            if (s.h_results.ensure(grow) != hipSuccess || s.d_results.ensure(grow) != hipSuccess) {
+        if (s.h_results.capacity() < rb) {
+            const size_t grow = std::max(rb, 2 * s.h_results.capacity());
+            if (s.h_results.reserve(grow) != hipSuccess || s.d_results.reserve(grow) != hipSuccess) {
                 return false;
             }
         }
-        if (digests_ != nullptr && s.h_digests.size < n * sizeof(uint64_t)) {
-            const size_t grow = std::max(n * sizeof(uint64_t), 2 * s.h_digests.size);
-            if (s.h_digests.ensure(grow) != hipSuccess || s.d_digests.ensure(grow) != hipSuccess) {
+        if (digests_ != nullptr && s.h_digests.capacity() < n * sizeof(uint64_t)) {
+            const size_t grow = std::max(n * sizeof(uint64_t), 2 * s.h_digests.capacity());
+            if (s.h_digests.reserve(grow) != hipSuccess || s.d_digests.reserve(grow) != hipSuccess) {
                 return false;
             }
         }
-        uint64_t* t = static_cast<uint64_t*>(s.h_tables.p);
+        uint64_t* t = static_cast<uint64_t*>(s.h_tables.data());
         uint64_t* t_src = t;
         uint64_t* t_size = t + n;
         uint64_t* t_dst = t + 2 * n;
         uint64_t* t_cap = t + 3 * n;
-        const uintptr_t base = reinterpret_cast<uintptr_t>(pool_mem_.p);
+        const uintptr_t base = reinterpret_cast<uintptr_t>(pool_mem_.data());
         size_t i = 0;
         for (size_t r : batch) {
             size_t off = state_[r].offset;
@@ -256,12 +262,12 @@ private:
                 compressed_bytes_ += sizes_[c];
             }
         }
-        const hipStream_t st = s.stream.s;
-        uint64_t* d = static_cast<uint64_t*>(s.d_tables.p);
+        const hipStream_t st = s.stream.get();
+        uint64_t* d = static_cast<uint64_t*>(s.d_tables.data());
         if (hipMemcpyAsync(d, t, want, hipMemcpyHostToDevice, st) != hipSuccess) {
             return false;
         }
-        auto* dres = static_cast<cudec_chunk_result*>(s.d_results.p);
+        auto* dres = static_cast<cudec_chunk_result*>(s.d_results.data());
         const auto cs = reinterpret_cast<cudec_stream_t>(st);
         if (cudec_lz4_decompress_batch(reinterpret_cast<const void* const*>(d), d + n,
                                        reinterpret_cast<void* const*>(d + 2 * n), d + 3 * n, n,
@@ -270,15 +276,15 @@ private:
         }
         if (digests_ != nullptr) {
             if (cudec_mi355x_digest_batch(reinterpret_cast<const void* const*>(d + 2 * n), nullptr,
-                                          dres, n, static_cast<uint64_t*>(s.d_digests.p),
+                                          dres, n, static_cast<uint64_t*>(s.d_digests.data()),
                                           cs) != CUDEC_OK ||
-                hipMemcpyAsync(s.h_digests.p, s.d_digests.p, n * sizeof(uint64_t),
+                hipMemcpyAsync(s.h_digests.data(), s.d_digests.data(), n * sizeof(uint64_t),
                                hipMemcpyDeviceToHost, st) != hipSuccess) {
                 return false;
             }
         }
-        if (hipMemcpyAsync(s.h_results.p, dres, rb, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipEventRecord(s.done.e, st) != hipSuccess) {
+        if (hipMemcpyAsync(s.h_results.data(), dres, rb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipEventRecord(s.done.get(), st) != hipSuccess) {
             return false;
         }
         s.requests = batch;
@@ -294,8 +300,8 @@ private:
     }
 
     void retire(Slot& s, double t, size_t* held) {
-        const auto* res = static_cast<const cudec_chunk_result*>(s.h_results.p);
-        const auto* dig = static_cast<const uint64_t*>(s.h_digests.p);
+        const auto* res = static_cast<const cudec_chunk_result*>(s.h_results.data());
+        const auto* dig = static_cast<const uint64_t*>(s.h_digests.data());
         size_t i = 0;
         for (size_t r : s.requests) {
             latency_[r] = t - reqs_[r].arrival_s;
@@ -323,7 +329,7 @@ private:
     uint64_t* digests_;
     double* latency_;
     OutputPool pool_;
-    DeviceBuffer pool_mem_;
+    DeviceRegion pool_mem_;
     std::vector<Request> state_;
     std::vector<std::unique_ptr<Slot>> slots_;
     uint64_t launches_ = 0, max_inflight_ = 0, max_batch_requests_ = 0, peak_pool_ = 0;

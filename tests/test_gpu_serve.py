@@ -102,3 +102,22 @@ def test_budget_smaller_than_the_load_still_completes(pool):
     run = serve.serve(d.data_ptr(), p, req, ids, budget=budget, inflight=4)
     assert run.status == 0 and np.all(run.results["status"] == 0)
     assert run.stats.peak_pool_bytes <= budget
+
+
+def test_zero_budget_with_zero_capacity_chunks_returns():
+    """A request whose chunks all have capacity 0 holds no pool bytes, so it
+    is admitted even with a zero budget: the loop must finish, and each
+    one-byte block {00} decodes to OK with 0 bytes (the empty LZ4 block)."""
+    from types import SimpleNamespace
+    n = 6
+    blocks = SimpleNamespace(comp_off=np.zeros(n, np.uint64), comp_size=np.ones(n, np.uint64),
+                             sizes=np.zeros(n, np.uint64))
+    d = torch.zeros(16, dtype=torch.uint8, device="cuda")
+    req = np.zeros(3, dtype=serve.REQUEST_DTYPE)
+    req["arrival_s"] = [0.0, 0.0, 0.001]
+    req["first_chunk"] = [0, 2, 4]
+    req["chunk_count"] = [2, 2, 2]
+    run = serve.serve(d.data_ptr(), blocks, req, np.arange(n), budget=0, inflight=2)
+    assert run.status == 0
+    assert np.all(run.results["status"] == 0) and np.all(run.results["bytes_written"] == 0)
+    assert run.stats.peak_pool_bytes == 0
