@@ -44,9 +44,12 @@ ABI_SYMBOLS = {
     "cudec_snappy.h": ["cudec_snappy_decompress_batch"],
     "cudec_mi355x.h": [
         "cudec_lz4_decompress_batch_plain", "cudec_snappy_decompress_batch_plain",
-        "cudec_mi355x_max_batch_chunks",
+        "cudec_mi355x_max_batch_chunks", "cudec_mi355x_cost_bin",
+        "cudec_mi355x_set_decode_plan", "cudec_mi355x_decode_plan",
         "cudec_mi355x_set_lz4_pipeline_threshold", "cudec_mi355x_lz4_pipeline_threshold",
-        "cudec_mi355x_autotune_lz4", "cudec_mi355x_digest_batch", "cudec_mi355x_serve_lz4",
+        "cudec_mi355x_set_snappy_pipeline_threshold", "cudec_mi355x_snappy_pipeline_threshold",
+        "cudec_mi355x_autotune_lz4", "cudec_mi355x_autotune_plan", "cudec_mi355x_digest_batch",
+        "cudec_mi355x_serve_lz4",
     ],
 }
 
@@ -98,10 +101,23 @@ def lib() -> ctypes.CDLL:
         L.cudec_mi355x_serve_lz4.argtypes = [_vp, _vp, _vp, _sz, _vp, _sz, _sz, ctypes.c_int,
                                              _vp, _vp, _vp, _vp]
         L.cudec_mi355x_serve_lz4.restype = ctypes.c_int
-        L.cudec_mi355x_set_lz4_pipeline_threshold.argtypes = [_sz]
-        L.cudec_mi355x_set_lz4_pipeline_threshold.restype = ctypes.c_int
-        L.cudec_mi355x_lz4_pipeline_threshold.argtypes = [ctypes.POINTER(_sz)]
-        L.cudec_mi355x_lz4_pipeline_threshold.restype = ctypes.c_int
+        for fmt in ("lz4", "snappy"):
+            setter = getattr(L, f"cudec_mi355x_set_{fmt}_pipeline_threshold")
+            setter.argtypes = [_sz]
+            setter.restype = ctypes.c_int
+            getter = getattr(L, f"cudec_mi355x_{fmt}_pipeline_threshold")
+            getter.argtypes = [ctypes.POINTER(_sz)]
+            getter.restype = ctypes.c_int
+        L.cudec_mi355x_cost_bin.argtypes = [_sz, _sz]
+        L.cudec_mi355x_cost_bin.restype = ctypes.c_int
+        L.cudec_mi355x_set_decode_plan.argtypes = [ctypes.c_int, _vp]
+        L.cudec_mi355x_set_decode_plan.restype = ctypes.c_int
+        L.cudec_mi355x_decode_plan.argtypes = [ctypes.c_int, _vp]
+        L.cudec_mi355x_decode_plan.restype = ctypes.c_int
+        L.cudec_mi355x_autotune_plan.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, _sz, _vp, _vp,
+                                                 _vp, ctypes.POINTER(ctypes.c_float),
+                                                 ctypes.POINTER(ctypes.c_float)]
+        L.cudec_mi355x_autotune_plan.restype = ctypes.c_int
         L.cudec_mi355x_autotune_lz4.argtypes = [_vp, _vp, _vp, _vp, _sz, _vp, _vp,
                                                 ctypes.POINTER(_sz), _vp, _vp]
         L.cudec_mi355x_autotune_lz4.restype = ctypes.c_int
@@ -200,6 +216,8 @@ def cudec_lz4f_decompress(frame: bytes, capacity: int):
 # ------------------------------------------------------------------ tuning
 
 TUNE_CANDIDATES = 9
+COST_BINS = 256
+FORMAT_CODE = {"lz4": 0, "snappy": 1}
 
 
 def set_lz4_pipeline_threshold(min_cost: int) -> int:
@@ -211,6 +229,46 @@ def lz4_pipeline_threshold():
     v = _sz(0)
     st = lib().cudec_mi355x_lz4_pipeline_threshold(ctypes.byref(v))
     return st, v.value
+
+
+def set_pipeline_threshold(fmt: str, min_cost: int) -> int:
+    return getattr(lib(), f"cudec_mi355x_set_{fmt}_pipeline_threshold")(min_cost)
+
+
+def pipeline_threshold(fmt: str):
+    """(status, min_cost) of the current device for format fmt."""
+    v = _sz(0)
+    st = getattr(lib(), f"cudec_mi355x_{fmt}_pipeline_threshold")(ctypes.byref(v))
+    return st, v.value
+
+
+def cost_bin(compressed_size: int, capacity: int) -> int:
+    return lib().cudec_mi355x_cost_bin(compressed_size, capacity)
+
+
+def set_decode_plan(fmt: str, lanes) -> int:
+    """lanes: COST_BINS lane counts (64 or 128), bin 0 the costliest."""
+    arr = (ctypes.c_uint16 * COST_BINS)(*[int(x) for x in lanes])
+    return lib().cudec_mi355x_set_decode_plan(FORMAT_CODE[fmt], arr)
+
+
+def decode_plan(fmt: str):
+    """(status, [lanes per bin]) of the current device for format fmt."""
+    arr = (ctypes.c_uint16 * COST_BINS)()
+    st = lib().cudec_mi355x_decode_plan(FORMAT_CODE[fmt], arr)
+    return st, list(arr)
+
+
+def autotune_plan(fmt: str, d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
+                  chunk_count, d_results, stream=None):
+    """Runs cudec_mi355x_autotune_plan: (status, [lanes per bin],
+    best threshold plan ms, final plan ms)."""
+    arr = (ctypes.c_uint16 * COST_BINS)()
+    ms_t, ms_p = ctypes.c_float(0), ctypes.c_float(0)
+    st = lib().cudec_mi355x_autotune_plan(FORMAT_CODE[fmt], d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                                          d_dst_capacities, chunk_count, d_results, stream, arr,
+                                          ctypes.byref(ms_t), ctypes.byref(ms_p))
+    return st, list(arr), ms_t.value, ms_p.value
 
 
 def autotune_lz4(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities, chunk_count,

@@ -11,6 +11,10 @@
 
 namespace cudec_amd {
 
+static_assert(kCostBins == CUDEC_MI355X_COST_BINS, "one bin count");
+static_assert(kFormatLz4 == CUDEC_MI355X_FORMAT_LZ4 && kFormatSnappy == CUDEC_MI355X_FORMAT_SNAPPY,
+              "one format numbering");
+
 /* Launch-form selector. */
 enum Form : int { kFormPlain = 0, kFormTuned = 1 };
 
@@ -105,22 +109,63 @@ cudec_status cudec_snappy_decompress_batch_plain(
                   stream);
 }
 
-cudec_status cudec_mi355x_set_lz4_pipeline_threshold(size_t min_cost) {
+static cudec_status set_threshold(int format, size_t min_cost) {
     (void)hipGetLastError();
-    return set_lz4_pipe_min_cost(min_cost) == hipSuccess ? CUDEC_OK : CUDEC_ERR_CUDA;
+    return set_pipe_min_cost(format, min_cost) == hipSuccess ? CUDEC_OK : CUDEC_ERR_CUDA;
 }
 
-cudec_status cudec_mi355x_lz4_pipeline_threshold(size_t* min_cost) {
+static cudec_status get_threshold(int format, size_t* min_cost) {
     if (min_cost == nullptr) {
         return CUDEC_ERR_INVALID_ARGUMENT;
     }
     (void)hipGetLastError();
     uint64_t v = 0;
-    if (get_lz4_pipe_min_cost(&v) != hipSuccess) {
+    if (get_pipe_min_cost(format, &v) != hipSuccess) {
         return CUDEC_ERR_CUDA;
     }
     *min_cost = (size_t)v;
     return CUDEC_OK;
+}
+
+cudec_status cudec_mi355x_set_lz4_pipeline_threshold(size_t min_cost) {
+    return set_threshold(kFormatLz4, min_cost);
+}
+
+cudec_status cudec_mi355x_lz4_pipeline_threshold(size_t* min_cost) {
+    return get_threshold(kFormatLz4, min_cost);
+}
+
+cudec_status cudec_mi355x_set_snappy_pipeline_threshold(size_t min_cost) {
+    return set_threshold(kFormatSnappy, min_cost);
+}
+
+cudec_status cudec_mi355x_snappy_pipeline_threshold(size_t* min_cost) {
+    return get_threshold(kFormatSnappy, min_cost);
+}
+
+int cudec_mi355x_cost_bin(size_t compressed_size, size_t capacity) {
+    return (int)host_cost_bin(compressed_size, capacity);
+}
+
+cudec_status cudec_mi355x_set_decode_plan(int format, const uint16_t* h_lanes_per_bin) {
+    if ((format != kFormatLz4 && format != kFormatSnappy) || h_lanes_per_bin == nullptr) {
+        return CUDEC_ERR_INVALID_ARGUMENT;
+    }
+    for (int b = 0; b < kCostBins; b++) {
+        if (h_lanes_per_bin[b] != 64 && h_lanes_per_bin[b] != 128) {
+            return CUDEC_ERR_INVALID_ARGUMENT;
+        }
+    }
+    (void)hipGetLastError();
+    return set_decode_plan(format, h_lanes_per_bin) == hipSuccess ? CUDEC_OK : CUDEC_ERR_CUDA;
+}
+
+cudec_status cudec_mi355x_decode_plan(int format, uint16_t* h_lanes_per_bin) {
+    if ((format != kFormatLz4 && format != kFormatSnappy) || h_lanes_per_bin == nullptr) {
+        return CUDEC_ERR_INVALID_ARGUMENT;
+    }
+    (void)hipGetLastError();
+    return get_decode_plan(format, h_lanes_per_bin) == hipSuccess ? CUDEC_OK : CUDEC_ERR_CUDA;
 }
 
 size_t cudec_mi355x_max_batch_chunks(void) { return kMaxBatchChunks; }

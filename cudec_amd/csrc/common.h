@@ -90,11 +90,19 @@ hipError_t launch_plain(int format, const void* const* srcs,
                         const size_t* caps, size_t n,
                         cudec_chunk_result* results, hipStream_t stream);
 
-/* The current device's LZ4 pipelining threshold (tuned.hip): chunks whose
- * cost (capacity + (capacity - compressed size)) is at least this many
- * bytes decode on two waves; 0 means every chunk, UINT64_MAX none. */
-hipError_t set_lz4_pipe_min_cost(uint64_t min_cost);
-hipError_t get_lz4_pipe_min_cost(uint64_t* min_cost);
+/* The current device's decode plan of one format (tuned.hip): for each of
+ * the 256 cost bins (cost = capacity + (capacity - compressed size), four
+ * bins per octave, bin 0 the costliest), the lanes that decode a chunk of
+ * that bin, 128 (two pipelined waves) or 64 (one wave). A threshold sets
+ * the plan that puts chunks costing at least min_cost on two waves (0:
+ * every chunk, UINT64_MAX none); get_pipe_min_cost reports the threshold
+ * last set. */
+constexpr int kCostBins = 256;
+hipError_t set_pipe_min_cost(int format, uint64_t min_cost);
+hipError_t get_pipe_min_cost(int format, uint64_t* min_cost);
+hipError_t set_decode_plan(int format, const uint16_t* lanes);
+hipError_t get_decode_plan(int format, uint16_t* lanes);
+uint32_t host_cost_bin(uint64_t src_size, uint64_t cap);
 
 }  // namespace cudec_amd
 

@@ -54,6 +54,35 @@ struct Lz4Step {
     uint32_t offset;
 };
 
+/* Where a ladder reads its bytes. GlobalBytes: the chunk's source in
+ * global memory. StagedBytes: a copy of src[base, base + avail) in LDS
+ * (a window's stage), falling back to global memory past it; the ladder
+ * only reads at or after base. Both are read with wave-uniform positions. */
+struct GlobalBytes {
+    gcu8* src;
+    __device__ __forceinline__ uint32_t byte(uint64_t p) const { return src[p]; }
+    __device__ __forceinline__ bool local(uint64_t, uint32_t) const { return false; }
+    __device__ __forceinline__ uint32_t word(uint64_t) const { return 0; }
+};
+struct StagedBytes {
+    gcu8* src;
+    const uint8_t* cw;
+    uint64_t base;
+    uint32_t avail;
+    __device__ __forceinline__ uint32_t byte(uint64_t p) const {
+        return p - base < avail ? (uint32_t)cw[p - base] : (uint32_t)src[p];
+    }
+    /* the n bytes at p are all in the stage */
+    __device__ __forceinline__ bool local(uint64_t p, uint32_t n) const {
+        return p - base < avail && avail - (p - base) >= n;
+    }
+    __device__ __forceinline__ uint32_t word(uint64_t p) const {
+        uint32_t v;
+        __builtin_memcpy(&v, cw + (p - base), 4);
+        return v;
+    }
+};
+
 /* 255-continued length extension (src/lz4_block.h:95-128). */
 __device__ __forceinline__ int lz4_accumulate(Lz4Cursor& c, uint64_t& length,
                                               uint64_t margin,
@@ -82,17 +111,92 @@ __device__ __forceinline__ int lz4_accumulate(Lz4Cursor& c, uint64_t& length,
     return kOk;
 }
 
-/* One sequence (src/lz4_block.h:139-258). */
+/* lz4_accumulate with the whole wave reading continuation bytes, 256 a
+ * round: lane i takes the four bytes at sp + 4 i (one dword load where all
+ * four lie in the source). Each lane evaluates the serial loop's checks for
+ * its bytes as that loop would meet them there (every earlier byte of the
+ * run is 255, so the length before byte k is length + 255 k), and the
+ * first byte in stream order at which the loop would leave decides, with
+ * the status the loop leaves with. Results equal lz4_accumulate's. Its fuel
+ * check never decides: it could first fire one byte after the one where sp
+ * reaches src_size, and that byte already returns. Called with the whole
+ * wave active and the cursor uniform. */
+template <class Bytes>
+__device__ __forceinline__ int lz4_accumulate_wave(Lz4Cursor& c, const Bytes& in, uint64_t& length,
+                                                   uint64_t margin, bool initial_check) {
+    if (initial_check && c.sp + margin >= c.src_size) {
+        return kCorrupt;
+    }
+    const uint32_t lane = __lane_id();
+    const uint64_t room = c.cap - c.dp;
+    constexpr int kNone = -1;
+    for (;;) {
+        const uint64_t p0 = c.sp + 4u * lane;
+        uint32_t bytes = 0;
+        if (p0 + 4 <= c.src_size && in.local(p0, 4)) {
+            bytes = in.word(p0);
+        } else if (p0 + 4 <= c.src_size) {
+            __builtin_memcpy(&bytes, (const void*)(c.src + p0), 4);
+        } else {
+            for (uint32_t u = 0; u < 4; u++) {
+                if (p0 + u < c.src_size) {
+                    bytes |= (uint32_t)c.src[p0 + u] << (8 * u);
+                }
+            }
+        }
+        int st = kNone;
+        uint32_t at = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            const uint64_t p = p0 + u;
+            const uint32_t byte = (bytes >> (8 * u)) & 0xFFu;
+            if (st == kNone) {
+                at = u;
+                if (p >= c.src_size) {
+                    st = kCorrupt;
+                } else if (length + 255u * (p - c.sp) > room) {
+                    st = kTooSmall;
+                } else if (p + 1 + margin > c.src_size) {
+                    st = kCorrupt;
+                } else if (byte != 255u) {
+                    st = kOk;
+                }
+            }
+        }
+        const uint64_t hit = __ballot(st != kNone);
+        if (hit != 0) {
+            const uint32_t i = (uint32_t)__builtin_ctzll(hit);
+            const int s = (int)uniform32(__builtin_amdgcn_readlane((uint32_t)st, i));
+            if (s == kOk) {
+                const uint32_t u = uniform32(__builtin_amdgcn_readlane(at, i));
+                const uint32_t word = uniform32(__builtin_amdgcn_readlane(bytes, i));
+                const uint64_t k = 4u * (uint64_t)i + u;
+                length += 255u * k + ((word >> (8 * u)) & 0xFFu);
+                c.sp += k + 1;
+            }
+            return s;
+        }
+        length += 255u * 256u;
+        c.sp += 256;
+    }
+}
+
+/* One sequence (src/lz4_block.h:139-258), its header bytes read from `in`.
+ * kWaveScan: length extensions are read by the whole wave
+ * (lz4_accumulate_wave), for callers that run the ladder with every lane
+ * active; otherwise byte by byte from global memory. */
+template <bool kWaveScan = false, class Bytes = GlobalBytes>
 __device__ __forceinline__ int lz4_next(Lz4Cursor& c, Lz4Step& q,
-                                        bool& done) {
+                                        bool& done, const Bytes& in) {
     done = false;
     if (c.sp >= c.src_size) {
         return kCorrupt;
     }
-    const uint32_t token = c.src[c.sp++];
+    const uint32_t token = in.byte(c.sp++);
     uint64_t lit = token >> 4;
     if (lit == kLz4RunMask) {
-        const int st = lz4_accumulate(c, lit, kLz4LitMargin, true);
+        const int st = kWaveScan ? lz4_accumulate_wave(c, in, lit, kLz4LitMargin, true)
+                                 : lz4_accumulate(c, lit, kLz4LitMargin, true);
         if (st != kOk) {
             return st;
         }
@@ -125,8 +229,7 @@ __device__ __forceinline__ int lz4_next(Lz4Cursor& c, Lz4Step& q,
     }
     c.sp += lit;
     c.dp += lit;
-    const uint32_t offset =
-        (uint32_t)c.src[c.sp] | ((uint32_t)c.src[c.sp + 1] << 8);
+    const uint32_t offset = in.byte(c.sp) | (in.byte(c.sp + 1) << 8);
     c.sp += 2;
     if (offset == 0) {
         return kCorrupt;
@@ -136,7 +239,8 @@ __device__ __forceinline__ int lz4_next(Lz4Cursor& c, Lz4Step& q,
     }
     uint64_t match = (token & kLz4RunMask) + kLz4MinMatch;
     if ((token & kLz4RunMask) == kLz4RunMask) {
-        const int st = lz4_accumulate(c, match, kLz4MatchMargin, false);
+        const int st = kWaveScan ? lz4_accumulate_wave(c, in, match, kLz4MatchMargin, false)
+                                 : lz4_accumulate(c, match, kLz4MatchMargin, false);
         if (st != kOk) {
             return st;
         }
@@ -152,6 +256,10 @@ __device__ __forceinline__ int lz4_next(Lz4Cursor& c, Lz4Step& q,
     q.offset = offset;
     c.dp += match;
     return kOk;
+}
+
+__device__ __forceinline__ int lz4_next(Lz4Cursor& c, Lz4Step& q, bool& done) {
+    return lz4_next<false>(c, q, done, GlobalBytes{c.src});
 }
 
 /* Snappy (src/snappy_block.h:58-435). */

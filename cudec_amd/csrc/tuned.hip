@@ -2,10 +2,13 @@
  *
  * Work distribution: four small kernels bin the chunks by decode cost
  * (capacity + (capacity - compressed size), the output plus what matches
- * add to it; four bins per octave) into a device queue, costliest first. A persistent grid of two-wave blocks then pulls chunks
- * from it. A chunk at or above the tuned cost threshold runs on both waves
- * of its block, as a software pipeline. Once the queue reaches cheaper
- * chunks, a block splits into two independent one-wave decoders.
+ * add to it; four bins per octave) into a device queue. A per-device,
+ * per-format decode plan names, for each cost bin, the lanes that decode
+ * one of its chunks: 128 (both waves of a block, as a software pipeline)
+ * or 64 (one wave). The queue holds the two-wave chunks first, then the
+ * one-wave ones, each class costliest first. A persistent grid of two-wave
+ * blocks pulls chunks from it; once the queue reaches the one-wave class,
+ * a block splits into two independent one-wave decoders.
  *
  * Per chunk, the stream is decoded in windows of candidate positions
  * staged in LDS:
@@ -66,15 +69,18 @@ constexpr uint32_t kCandPipeSnappy = 7;
 constexpr uint32_t stage_bytes(uint32_t kc) { return kc * 64u + 512u; }
 constexpr uint32_t kStageBuf = stage_bytes(kCandPipe) + 16;
 constexpr uint32_t kRing = 4096;          /* per-wave output ring */
-constexpr uint32_t kBatchMax = 2048;      /* output bytes of one batch */
+constexpr uint32_t kBatchMax = 2048;      /* output bytes of one batch, one-wave form */
 constexpr uint32_t kStop = 0xFFFFu;       /* link of an unresolvable candidate */
 constexpr uint32_t kLaneLit = 64;         /* literal runs copied by their lane */
 constexpr uint32_t kLaneMatch = 64;       /* matches copied by their lane (<= 64) */
 constexpr uint32_t kOctaveBits = 2;       /* cost bins per octave: 4 */
 constexpr int kBins = 64 << kOctaveBits;
-/* Default: chunks costing at least 512 KiB (capacity + capacity - compressed) run
- * on two waves. */
+static_assert(kBins == kCostBins, "the plan API names every bin");
+/* Default plan: chunks costing at least 512 KiB (capacity + capacity -
+ * compressed) run on two waves. */
 constexpr uint64_t kDefaultPipeMinCost = 512u << 10;
+/* Queue keys: a chunk's class (two-wave first), then its cost bin. */
+constexpr int kKeys = 2 * kBins;
 
 static_assert(kBatchMax + 32 <= kRing, "ring must hold a batch and a margin");
 static_assert((kRing & (kRing - 1)) == 0, "rings are powers of two");
@@ -99,6 +105,14 @@ struct alignas(16) WaveLds : RingLds<kRing> {
 };
 constexpr uint32_t kPipeRing = 8192;
 using PipeRing = RingLds<kPipeRing>;
+/* Output bytes of one batch in the two-wave form. Windows of token data
+ * produce under 2 KiB; a window of long short-offset matches (64 members
+ * of about 90 bytes each) fills up to 6 KiB, which its 8 KiB ring holds
+ * with 2 KiB of history. */
+constexpr uint32_t kBatchMaxPipe = 6144;
+static_assert(kBatchMaxPipe + 32 <= kPipeRing, "the two-wave ring holds a batch and a margin");
+/* a batch's limit: the one-wave form walks by pointer doubling */
+constexpr uint32_t batch_max(bool one_wave) { return one_wave ? kBatchMax : kBatchMaxPipe; }
 
 /* The decode queue hands out positions of the cost-sorted chunk order in
  * up to kSubQueues interleaved sub-queues (of k: sub-queue s holds
@@ -121,10 +135,20 @@ struct QueueCursor {
 /* Scratch of one tuned launch (stream-ordered allocation). */
 struct Queue {
     uint32_t next[kSubQueues * kSubStride];
-    uint32_t bin_count[kBins];
-    uint32_t bin_base[kBins];
-    uint32_t bin_cursor[kBins];
+    uint32_t bin_count[kKeys];
+    uint32_t bin_base[kKeys];
+    uint32_t bin_cursor[kKeys];
 };
+
+/* A decode plan: bit b (of word b / 64) set when cost bin b decodes on
+ * two pipelined waves, clear for one wave. Passed to the kernels by value. */
+struct Plan {
+    uint64_t two_wave[kBins / 64];
+};
+
+__host__ __device__ inline bool plan_two_wave(const Plan& p, uint32_t bin) {
+    return ((p.two_wave[bin >> 6] >> (bin & 63)) & 1u) != 0;
+}
 
 template <uint32_t kR = kRing>
 __device__ __forceinline__ uint32_t ring_index(uintptr_t dst_addr, uint64_t pos) {
@@ -186,7 +210,10 @@ struct Cand {
 };
 
 /* The sequence a token at window position c would encode, assuming at most
- * one extension byte per length. Reads stay below c + 275, inside the stage. */
+ * one extension byte per length. Reads stay below c + 275, inside the
+ * stage. (Windows that also held matches with two extension bytes, up to
+ * 529 bytes, decoded runs of long short-offset matches 20% faster but token
+ * data 2.4% slower: the wider parse cost the kernel registers it spilled.) */
 __device__ __forceinline__ Cand parse_candidate(const uint8_t* cw, uint32_t c) {
     /* branch-free: the token and the byte after it, then the three bytes
      * at the offset (offset, match extension); bytes past the sequence are
@@ -514,7 +541,23 @@ __device__ __forceinline__ void wave_copy_global(gu8* dst, gcu8* src,
         dst[lane] = src[lane];
     }
     const uint64_t body = (n - head) / 16;
-    for (uint64_t b = lane; b < body; b += kWave) {
+    /* eight loads a lane in flight before their stores: one wave keeps
+     * 8 KiB moving, where a load-then-store step waits a full memory
+     * latency per KiB (a 4 MiB literal run took ~5 ms that way) */
+    constexpr uint32_t kDeep = 4;
+    uint64_t b = lane;
+    for (; b + (kDeep - 1) * kWave < body; b += kDeep * kWave) {
+        uint4 v[kDeep];
+#pragma unroll
+        for (uint32_t u = 0; u < kDeep; u++) {
+            v[u] = gload16(src + head + 16 * (b + u * kWave));
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kDeep; u++) {
+            gstore16(dst + head + 16 * (b + u * kWave), v[u]);
+        }
+    }
+    for (; b < body; b += kWave) {
         gstore16(dst + head + 16 * b, gload16(src + head + 16 * b));
     }
     const uint64_t done = head + 16 * body;
@@ -616,6 +659,104 @@ __device__ __forceinline__ uint32_t lane_mod(uint32_t lane, float lane_f, uint32
     const float inv = __builtin_amdgcn_rcpf((float)d);
     const uint32_t r = lane - __umul24(d, (uint32_t)(lane_f * inv));
     return r >= d ? r - d : r;
+}
+
+/* 16 ring bytes from index r, wrapping at the ring's end. */
+template <class W>
+__device__ __forceinline__ uint4 ring_read16(const W& w, uint32_t r) {
+    uint4 v;
+    if (r <= W::kSize - 16) {
+        __builtin_memcpy(&v, &w.ring[r], 16);
+    } else {
+        uint32_t q[4] = {0, 0, 0, 0};
+        for (uint32_t i = 0; i < 16; i++) {
+            q[i >> 2] |= (uint32_t)w.ring[(r + i) & (W::kSize - 1)] << ((i & 3) * 8);
+        }
+        v = make_uint4(q[0], q[1], q[2], q[3]);
+    }
+    return v;
+}
+
+/* Bytes t < keep of a, the rest of b. */
+__device__ __forceinline__ uint4 take_bytes(const uint4& a, const uint4& b, uint32_t keep) {
+    if (keep >= 16) {
+        return a;
+    }
+    uint32_t m[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+        const uint32_t c = keep > 4 * q ? keep - 4 * q : 0;
+        m[q] = c >= 4 ? 0xFFFFFFFFu : (1u << (8 * c)) - 1u;
+    }
+    return make_uint4((a.x & m[0]) | (b.x & ~m[0]), (a.y & m[1]) | (b.y & ~m[1]),
+                      (a.z & m[2]) | (b.z & ~m[2]), (a.w & m[3]) | (b.w & ~m[3]));
+}
+
+/* x mod d for x < 4096 and d >= 1, by the float reciprocal as in lane_mod
+ * (the quotient's error stays below one part in 2^11, so it falls short of
+ * the true one by at most one, and only at exact multiples). */
+__device__ __forceinline__ uint32_t small_mod(uint32_t x, uint32_t d) {
+    const float inv = __builtin_amdgcn_rcpf((float)d);
+    const uint32_t r = x - __umul24(d, (uint32_t)((float)x * inv));
+    return r >= d ? r - d : r;
+}
+
+/* The match [m, m + len) with offset o, whole wave, 16 bytes a lane a round
+ * of 1024, into the ring, when its period [m - o, m - o + min(o, len))
+ * lies in the ring. Match byte i is period byte i mod o, so every lane
+ * reads only the period, never bytes this match writes, and the lanes are
+ * independent. Lane k's block starts at match byte i0 = 16 k (+ 1024 a
+ * round), at period phase s = i0 mod o:
+ *  - o >= 16: the block is period bytes s, s + 1, ..., wrapping at most
+ *    once, i.e. ring bytes from m - o + s while t < o - s, and from one
+ *    period further back, m - 2 o + s, after that: two 16-byte reads and a
+ *    byte mask;
+ *  - o < 16: the first 16 ring bytes from m - o hold the period, and byte
+ *    t is its byte (s + t) mod o, gathered with v_perm selectors (periods
+ *    1 and 2 are widened to 4 first).
+ * Each block stores exactly its bytes of the match. */
+template <class W>
+__device__ __forceinline__ void ring_match_wide(W& w, uintptr_t da, uint64_t m, uint32_t o,
+                                                uint32_t len, uint32_t lane) {
+    const uint32_t rm = ring_index<W::kSize>(da, m);
+    const uint32_t rp = ring_index<W::kSize>(da, m - o);
+    uint32_t s = small_mod(16 * lane, o);
+    const uint32_t step = small_mod(16 * kWave, o);
+    if (o >= 16) {
+        for (uint32_t i0 = 16 * lane; i0 < len; i0 += 16 * kWave) {
+            const uint4 a = ring_read16(w, (rp + s) & (W::kSize - 1));
+            const uint4 b = ring_read16(w, (rp + s - o) & (W::kSize - 1));
+            const uint32_t n = len - i0 < 16 ? len - i0 : 16;
+            ring_put(w, (rm + i0) & (W::kSize - 1), take_bytes(a, b, o - s), n);
+            s += step;
+            s = s >= o ? s - o : s;
+        }
+        return;
+    }
+    uint4 S = ring_read16(w, rp);
+    uint32_t p = o;
+    if (p <= 2) {
+        /* periods 1 and 2 as period 4 */
+        S.x = __builtin_amdgcn_perm(0u, S.x, p == 1 ? 0x00000000u : 0x01000100u);
+        p = 4;
+    }
+    const uint32_t pb = __builtin_amdgcn_perm(0u, p, 0u);
+    const uint32_t cp = __builtin_amdgcn_perm(0u, 0x80u - p, 0u);
+    for (uint32_t i0 = 16 * lane; i0 < len; i0 += 16 * kWave) {
+        uint32_t q[4];
+        uint32_t r = s;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            q[t] = gather16(S, period_sel(r, pb, cp));
+            r += 4;
+            r = r >= p ? r - p : r;
+            r = r >= p ? r - p : r;
+        }
+        const uint32_t n = len - i0 < 16 ? len - i0 : 16;
+        ring_put(w, (rm + i0) & (W::kSize - 1), make_uint4(q[0], q[1], q[2], q[3]), n);
+        s += step;
+        s = s >= o ? s - o : s;
+    }
 }
 
 /* The members of one window that a group executes, as the copy phase needs
@@ -1009,6 +1150,9 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
     out.delta = stage_into<kC>(stage, src, src_size, sp, lane);
     touch = touch_ahead<kC>(src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
+    /* the scalar ladder reads a sequence's header from the stage while it
+     * lies there (an LDS round trip per byte instead of a global one) */
+    const StagedBytes staged{src, cw, base, stage_bytes(kC) + 16 - out.delta};
 
     /* speculate: the link of each candidate position */
     uint32_t link[kC];
@@ -1025,7 +1169,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
     if (uniform32(__builtin_amdgcn_readlane(link[0], 0)) == kStop) {
         Lz4Cursor cur{src, src_size, cap, sp, dp};
         bool last = false;
-        const int st = (int)uniform32((uint32_t)lz4_next(cur, out.q, last));
+        const int st = (int)uniform32((uint32_t)lz4_next<true>(cur, out.q, last, staged));
         if (st != kOk) {
             out.status = st;
             return out;
@@ -1127,7 +1271,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
             }
         }
         const uint32_t mine = terminal ? c.lit_len : out_len;
-        const bool cut = is_member && drel + mine > kBatchMax;
+        const bool cut = is_member && drel + mine > batch_max(kDoubling);
         const bool flag = is_member && (cut || ev != kOk || terminal);
         const uint64_t flags = __ballot(flag);
         uint32_t take = count; /* members executed in full */
@@ -1185,7 +1329,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
     /* one sequence by the exact scalar ladder */
     Lz4Cursor cur{src, src_size, cap, sp, dp};
     bool last = false;
-    const int st = (int)uniform32((uint32_t)lz4_next(cur, out.q, last));
+    const int st = (int)uniform32((uint32_t)lz4_next<true>(cur, out.q, last, staged));
     if (st != kOk) {
         out.status = st;
         return out;
@@ -1393,7 +1537,7 @@ __device__ __forceinline__ Parsed parse_window_snappy(uint8_t* stage, uint16_t* 
                 }
             }
         }
-        const bool cut = is_member && !at_end && drel + out_len > kBatchMax;
+        const bool cut = is_member && !at_end && drel + out_len > batch_max(kDoubling);
         const bool flag = is_member && (cut || ev != kOk || at_end);
         const uint64_t flags = __ballot(flag);
         uint32_t take = count;
@@ -1495,22 +1639,75 @@ __device__ __forceinline__ int chunk_begin(gcu8* src, uint64_t src_size, uint64_
     return st;
 }
 
-/* Executes a sequence the window could not hold: everything in the ring
- * goes out first, then the literal and the match run global to global. */
+/* Largest offset whose period a slow match takes from the ring. */
+constexpr uint32_t kSlowRingPeriod = 1024;
+
+/* Executes a sequence the window could not hold (a length with two or more
+ * extension bytes, a literal past the window). A literal of at most 64
+ * bytes goes into the ring a byte a lane; a longer one runs global to
+ * global after the ring is flushed up to it. A match whose period (offset
+ * <= kSlowRingPeriod) is in the ring is expanded there 16 bytes a lane
+ * (ring_match_wide), in pieces that never lap unflushed bytes or the
+ * period, with the finished blocks flushed before each piece; any other
+ * match runs global to global (wave_match_global), after everything before
+ * it is flushed. On return fewer than 16 bytes are unflushed (none when
+ * `done`: the chunk ends here). */
 template <class W>
 __device__ __forceinline__ void execute_slow(W& w, gcu8* src, gu8* dst,
                                              uint64_t cap, const Lz4Step& q, uint64_t dp_after,
                                              uint64_t& dp,
-                                             uint64_t& flushed, uint64_t& ring_lo,
+                                             uint64_t& flushed, uint64_t& ring_lo, bool done,
                                              uint32_t lane) {
-    flush_ring(w, dst, flushed, dp, lane);
-    wave_copy_global(dst + q.lit_dst, src + q.lit_src, q.lit_len, lane);
-    if (q.match_len != 0) {
-        wave_match_global(w, dst, q.match_dst, q.offset, q.match_len, cap, lane);
+    const uintptr_t da = (uintptr_t)dst;
+    if (q.lit_len <= kWave) {
+        if (lane < q.lit_len) {
+            w.ring[ring_index<W::kSize>(da, q.lit_dst + lane)] = src[q.lit_src + lane];
+        }
+        wave_order();
+    } else {
+        flush_ring(w, dst, flushed, q.lit_dst, lane);
+        wave_copy_global(dst + q.lit_dst, src + q.lit_src, q.lit_len, lane);
+        flushed = q.lit_dst + q.lit_len;
+        ring_lo = flushed; /* the ring holds none of it */
     }
+    if (q.match_len != 0) {
+        uint64_t m = q.match_dst;
+        const uint32_t o = q.offset;
+        const uint64_t floor = m + 16 > W::kSize ? m + 16 - W::kSize : 0;
+        if (o <= kSlowRingPeriod && m - o >= (ring_lo > floor ? ring_lo : floor)) {
+            /* a piece leaves the period and the (< 16) unflushed bytes
+             * below m in place */
+            constexpr uint32_t kPiece = W::kSize - kSlowRingPeriod - 32;
+            uint64_t left = q.match_len;
+            while (left != 0) {
+                /* whole blocks below m (compared as addresses: near the
+                 * chunk's start the aligned address can precede dst) */
+                const uintptr_t a_to = (da + m) & ~(uintptr_t)15;
+                if (a_to > da + flushed) {
+                    flush_ring(w, dst, flushed, (uint64_t)(a_to - da), lane);
+                    flushed = (uint64_t)(a_to - da);
+                }
+                const uint32_t piece = left < kPiece ? (uint32_t)left : kPiece;
+                ring_match_wide(w, da, m, o, piece, lane);
+                wave_order();
+                m += piece;
+                left -= piece;
+            }
+        } else {
+            flush_ring(w, dst, flushed, m, lane);
+            wave_match_global(w, dst, m, o, q.match_len, cap, lane);
+            flushed = m + q.match_len;
+            ring_lo = flushed; /* wave_match_global used the ring as scratch */
+        }
+    }
+    /* leave fewer than 16 unflushed bytes (none when the chunk is done):
+     * the next group's output may fill the ring up to its batch limit */
     dp = dp_after;
-    flushed = dp;
-    ring_lo = dp;
+    const uintptr_t a_end = done ? da + dp : ((da + dp) & ~(uintptr_t)15);
+    if (a_end > da + flushed) {
+        flush_ring(w, dst, flushed, (uint64_t)(a_end - da), lane);
+        flushed = (uint64_t)(a_end - da);
+    }
 }
 
 /* ------------------------------------------------- two-wave pipeline */
@@ -1550,7 +1747,7 @@ struct alignas(16) PipeLds {
     uint32_t pad[2];
 };
 
-static_assert(kLaneLit < 65536 && kBatchMax < 65536 && kStageBuf < 65536,
+static_assert(kLaneLit < 65536 && kBatchMaxPipe < 65536 && kStageBuf < 65536,
               "window records hold 16-bit fields");
 
 /* ------------------------------------------------------------- LZ4 */
@@ -1583,12 +1780,13 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
         }
         if (pr.kind == kParseGroup) {
             if (pr.lead) {
-                execute_slow(w, src, dst, cap, pr.q, pr.lead_dp, dp, flushed, ring_lo, lane);
+                execute_slow(w, src, dst, cap, pr.q, pr.lead_dp, dp, flushed, ring_lo, false,
+                             lane);
             }
             execute_group(w, w.stage + pr.delta, pr.g, dst, cap, dp, flushed, ring_lo, pr.done,
                           lane);
         } else {
-            execute_slow(w, src, dst, cap, pr.q, pdp, dp, flushed, ring_lo, lane);
+            execute_slow(w, src, dst, cap, pr.q, pdp, dp, flushed, ring_lo, pr.done, lane);
         }
         /* the look-ahead loads land by now */
         asm volatile("" ::"v"(touch));
@@ -1606,7 +1804,7 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
 
 __global__ void queue_reset(Queue* q) {
     const uint32_t t = threadIdx.x;
-    if (t < kBins) {
+    if (t < kKeys) {
         q->bin_count[t] = 0;
         q->bin_cursor[t] = 0;
     }
@@ -1642,21 +1840,28 @@ __device__ __forceinline__ uint32_t cost_bin(uint64_t src_size, uint64_t cap) {
     return kBins - 1 - cost_key(cost);
 }
 
+/* Queue key of a chunk: its class under the plan, then its bin. */
+__device__ __forceinline__ uint32_t queue_key(const Plan& plan, uint64_t src_size, uint64_t cap) {
+    const uint32_t bin = cost_bin(src_size, cap);
+    return plan_two_wave(plan, bin) ? bin : kBins + bin;
+}
+
 /* Chunks per cost bin. A block counts its chunks in LDS and adds one
  * global atomic per bin it holds: a batch of like chunks (one bin) would
  * otherwise serialise one atomic per chunk on that bin's counter. */
-__global__ void queue_count(const size_t* sizes, const size_t* caps, size_t n, Queue* q) {
-    __shared__ uint32_t local[kBins];
-    for (uint32_t t = threadIdx.x; t < kBins; t += blockDim.x) {
+__global__ void queue_count(const size_t* sizes, const size_t* caps, size_t n, Queue* q,
+                            Plan plan) {
+    __shared__ uint32_t local[kKeys];
+    for (uint32_t t = threadIdx.x; t < kKeys; t += blockDim.x) {
         local[t] = 0;
     }
     __syncthreads();
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        atomicAdd(&local[cost_bin(sizes[i], caps[i])], 1u);
+        atomicAdd(&local[queue_key(plan, sizes[i], caps[i])], 1u);
     }
     __syncthreads();
-    for (uint32_t t = threadIdx.x; t < kBins; t += blockDim.x) {
+    for (uint32_t t = threadIdx.x; t < kKeys; t += blockDim.x) {
         if (local[t] != 0) {
             atomicAdd(&q->bin_count[t], local[t]);
         }
@@ -1666,7 +1871,7 @@ __global__ void queue_count(const size_t* sizes, const size_t* caps, size_t n, Q
 __global__ void queue_scan(Queue* q) {
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
-        for (int b = 0; b < kBins; b++) {
+        for (int b = 0; b < kKeys; b++) {
             q->bin_base[b] = acc;
             acc += q->bin_count[b];
         }
@@ -1677,21 +1882,21 @@ __global__ void queue_scan(Queue* q) {
  * that bin (one global atomic per bin the block holds) and its rank in the
  * block (an LDS atomic). */
 __global__ void queue_fill(const size_t* sizes, const size_t* caps, size_t n, Queue* q,
-                           uint32_t* order) {
-    __shared__ uint32_t local[kBins];
-    __shared__ uint32_t range[kBins];
-    for (uint32_t t = threadIdx.x; t < kBins; t += blockDim.x) {
+                           uint32_t* order, Plan plan) {
+    __shared__ uint32_t local[kKeys];
+    __shared__ uint32_t range[kKeys];
+    for (uint32_t t = threadIdx.x; t < kKeys; t += blockDim.x) {
         local[t] = 0;
     }
     __syncthreads();
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t b = 0, rank = 0;
     if (i < n) {
-        b = cost_bin(sizes[i], caps[i]);
+        b = queue_key(plan, sizes[i], caps[i]);
         rank = atomicAdd(&local[b], 1u);
     }
     __syncthreads();
-    for (uint32_t t = threadIdx.x; t < kBins; t += blockDim.x) {
+    for (uint32_t t = threadIdx.x; t < kKeys; t += blockDim.x) {
         if (local[t] != 0) {
             range[t] = atomicAdd(&q->bin_cursor[t], local[t]);
         }
@@ -1749,14 +1954,12 @@ template <int kFormat, int kWaves, uint32_t kSubs>
 __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu(kWaves)))
     piped_batch(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
                 const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
-                const uint32_t* order, int32_t pipe_bin) {
+                const uint32_t* order, Plan plan) {
     __shared__ PipeLds L;
     const uint32_t lane = __lane_id();
     const bool parser = uniform32(threadIdx.x / kWave) == 0;
     uint32_t split_idx = 0xFFFFFFFFu; /* first chunk decoded one wave per chunk */
     QueueCursor cursor{blockIdx.x % kSubs, kSubs};
-    /* pipe_bin: chunks whose cost bin is at most this run on two waves; -1:
-     * none (the first chunk splits the block) */
     for (;;) {
         /* both waves are done reading the previous chunk's control words
          * (chunk_idx, fin) before the parser writes the next chunk's */
@@ -1775,9 +1978,9 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
         gu8* dst = global_bytes(dsts[chunk]);
         const uint64_t src_size = uniform64(src_sizes[chunk]);
         const uint64_t cap = uniform64(caps[chunk]);
-        if ((int32_t)cost_bin(src_size, cap) > pipe_bin) {
-            /* the queue runs largest cost first: every chunk from here on
-             * is cheaper too, so the block splits for good */
+        if (!plan_two_wave(plan, cost_bin(src_size, cap))) {
+            /* the queue holds every two-wave chunk before the one-wave
+             * ones, so the block splits for good */
             split_idx = idx;
             break;
         }
@@ -1853,7 +2056,8 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                         s.offset = (uint32_t)uniform64(r.offset_s);
                         s.match_len = uniform64(r.match_len_s);
                         execute_slow(L.pipe.w, src, dst, cap, s, uniform64(r.dp_after), dp, flushed,
-                                     ring_lo, lane);
+                                     ring_lo, kind == kParseSlow && uniform32(r.done) != 0,
+                                     lane);
                     }
                     if (kind == kParseGroup) {
                         Group g;
@@ -1908,23 +2112,25 @@ struct DeviceState {
     int cus = 0;
     int pipe_blocks_per_cu = 0;  /* piped_batch<_, 4> (two-wave blocks) */
     int split_blocks_per_cu = 0; /* piped_batch<_, 5>: no two-wave chunk, or Snappy */
-    uint64_t lz4_pipe_min_cost = kDefaultPipeMinCost; /* two waves at or above this cost */
+    Plan plan[2];                /* per format (kFormatLz4, kFormatSnappy) */
+    uint64_t min_cost[2];        /* the threshold each plan was last set from */
     hipMemPool_t pool = nullptr;
     bool ready = false;
 };
 constexpr int kMaxDevices = 64;
 
-/* The cost bin at or below which (costlier chunks) a chunk runs on two
- * waves: the bin of a chunk whose cost is exactly min_cost (host mirror of
- * cost_bin). 0 admits every bin; UINT64_MAX none (-1). */
-int32_t pipe_bin_for(uint64_t min_cost) {
-    if (min_cost == 0) {
-        return kBins - 1;
+/* The plan that puts chunks costing at least min_cost on two waves: every
+ * bin up to the bin of a chunk whose cost is exactly min_cost (costlier
+ * bins have smaller numbers). 0 takes every bin, UINT64_MAX none. */
+Plan plan_from_threshold(uint64_t min_cost) {
+    Plan p = {};
+    const int32_t last = min_cost == 0 ? kBins - 1
+                         : min_cost == UINT64_MAX ? -1
+                                                  : (int32_t)(kBins - 1 - cost_key(min_cost));
+    for (int32_t b = 0; b <= last; b++) {
+        p.two_wave[b >> 6] |= 1ull << (b & 63);
     }
-    if (min_cost == UINT64_MAX) {
-        return -1;
-    }
-    return (int32_t)(kBins - 1 - cost_key(min_cost));
+    return p;
 }
 DeviceState g_devices[kMaxDevices];
 std::mutex g_devices_lock;
@@ -1958,6 +2164,10 @@ hipError_t device_state(DeviceState** out) {
             return e;
         }
         d.split_blocks_per_cu = per > 0 ? per : 1;
+        for (int f = 0; f < 2; f++) {
+            d.plan[f] = plan_from_threshold(kDefaultPipeMinCost);
+            d.min_cost[f] = kDefaultPipeMinCost;
+        }
         hipMemPoolProps props = {};
         props.allocType = hipMemAllocationTypePinned;
         props.location.type = hipMemLocationTypeDevice;
@@ -1980,26 +2190,69 @@ hipError_t device_state(DeviceState** out) {
 
 }  // namespace
 
-hipError_t set_lz4_pipe_min_cost(uint64_t min_cost) {
+hipError_t set_pipe_min_cost(int format, uint64_t min_cost) {
     DeviceState* dev = nullptr;
     const hipError_t e = device_state(&dev);
     if (e != hipSuccess) {
         return e;
     }
     std::lock_guard<std::mutex> hold(g_devices_lock);
-    dev->lz4_pipe_min_cost = min_cost;
+    dev->plan[format] = plan_from_threshold(min_cost);
+    dev->min_cost[format] = min_cost;
     return hipSuccess;
 }
 
-hipError_t get_lz4_pipe_min_cost(uint64_t* min_cost) {
+hipError_t get_pipe_min_cost(int format, uint64_t* min_cost) {
     DeviceState* dev = nullptr;
     const hipError_t e = device_state(&dev);
     if (e != hipSuccess) {
         return e;
     }
     std::lock_guard<std::mutex> hold(g_devices_lock);
-    *min_cost = dev->lz4_pipe_min_cost;
+    *min_cost = dev->min_cost[format];
     return hipSuccess;
+}
+
+hipError_t set_decode_plan(int format, const uint16_t* lanes) {
+    Plan p = {};
+    for (int b = 0; b < kBins; b++) {
+        if (lanes[b] == 128) {
+            p.two_wave[b >> 6] |= 1ull << (b & 63);
+        }
+    }
+    DeviceState* dev = nullptr;
+    const hipError_t e = device_state(&dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    std::lock_guard<std::mutex> hold(g_devices_lock);
+    dev->plan[format] = p;
+    return hipSuccess;
+}
+
+hipError_t get_decode_plan(int format, uint16_t* lanes) {
+    DeviceState* dev = nullptr;
+    const hipError_t e = device_state(&dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    Plan p;
+    {
+        std::lock_guard<std::mutex> hold(g_devices_lock);
+        p = dev->plan[format];
+    }
+    for (int b = 0; b < kBins; b++) {
+        lanes[b] = plan_two_wave(p, (uint32_t)b) ? 128 : 64;
+    }
+    return hipSuccess;
+}
+
+uint32_t host_cost_bin(uint64_t src_size, uint64_t cap) {
+    const uint64_t lim = 1ull << 61;
+    const uint64_t a = src_size < lim ? src_size : lim;
+    const uint64_t b = cap < lim ? cap : lim;
+    const uint64_t cost = b + (b > a ? b - a : 0);
+    return kBins - 1 - cost_key(cost);
 }
 
 hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_sizes,
@@ -2015,10 +2268,10 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     if (e != hipSuccess) {
         return e;
     }
-    uint64_t min_cost = 0;
+    Plan plan;
     {
         std::lock_guard<std::mutex> hold(g_devices_lock);
-        min_cost = dev->lz4_pipe_min_cost;
+        plan = dev->plan[format == kFormatSnappy ? kFormatSnappy : kFormatLz4];
     }
     const size_t bytes = sizeof(Queue) + sizeof(uint32_t) * n;
     /* under stream capture the scratch becomes a graph allocation node of
@@ -2039,16 +2292,20 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     uint32_t* order = reinterpret_cast<uint32_t*>(q + 1);
     const unsigned tb = 256;
     const unsigned nb = (unsigned)((n + tb - 1) / tb);
-    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(kBins), 0, stream, q);
-    hipLaunchKernelGGL(queue_count, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q);
+    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(kKeys), 0, stream, q);
+    hipLaunchKernelGGL(queue_count, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q, plan);
     hipLaunchKernelGGL(queue_scan, dim3(1), dim3(1), 0, stream, q);
-    hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q, order);
-    const int32_t pipe_bin = pipe_bin_for(min_cost);
+    hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, src_sizes, caps, n, q, order,
+                       plan);
+    bool any_two_wave = false;
+    for (int w = 0; w < kBins / 64; w++) {
+        any_two_wave = any_two_wave || plan.two_wave[w] != 0;
+    }
     /* two-wave blocks; a block splits into two one-wave decoders when it
      * meets a chunk below the threshold. The five-wave build runs where no
      * chunk is two-wave (every block splits at once) and for Snappy, whose
      * two-wave path fits its 96 VGPRs. */
-    const bool five_waves = pipe_bin < 0 || format == kFormatSnappy;
+    const bool five_waves = !any_two_wave || format == kFormatSnappy;
     size_t grid = (size_t)dev->cus *
                   (size_t)(five_waves ? dev->split_blocks_per_cu : dev->pipe_blocks_per_cu);
     if (grid > n) {
@@ -2061,7 +2318,7 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
 #define CUDEC_PIPED(kShape, kFmt, kW, kS)                                                   \
     case kShape:                                                                            \
         hipLaunchKernelGGL((piped_batch<kFmt, kW, kS>), gd, bd, 0, stream, srcs, src_sizes, \
-                           dsts, caps, n, results, q, order, pipe_bin);                     \
+                           dsts, caps, n, results, q, order, plan);                         \
         break;
         CUDEC_PIPED(0, kFormatLz4, 4, 1)
         CUDEC_PIPED(1, kFormatLz4, 4, kSubQueues)

@@ -38,19 +38,43 @@ cudec_status cudec_snappy_decompress_batch_plain(
  * src/batch_limits.h:28 for wave64: INT32_MAX * 64). */
 size_t cudec_mi355x_max_batch_chunks(void);
 
-/* No counterpart in the reference: launch tuning of the LZ4 batch entry
- * (cudec_lz4_decompress_batch) on the current device. Inside one launch the
- * chunks run costliest first (cost = capacity + (capacity - compressed
- * size): the output plus what matches add to it); a chunk
- * whose cost is at least `min_cost` bytes decodes on two waves (one
- * parses the next window while the other executes the current one), a
- * cheaper one on a single wave. 0 puts every chunk on two waves, SIZE_MAX
- * none. Output bytes and results never depend on this setting. Default:
- * 512 KiB. Settings are per device and process-wide. */
+/* No counterpart in the reference: launch tuning of the tuned batch
+ * entries (cudec_lz4_decompress_batch, cudec_snappy_decompress_batch) on
+ * the current device. A chunk's cost is capacity + (capacity - compressed
+ * size): the output, plus what matches add to it. Costs fall into
+ * CUDEC_MI355X_COST_BINS bins, four per octave, bin 0 the costliest
+ * (cudec_mi355x_cost_bin). A format's decode plan names, for each bin, the
+ * lanes that decode one of its chunks: 128 (two waves of a block, one
+ * parsing the next window while the other executes the current one) or 64
+ * (one wave). Inside one launch the two-wave chunks start first, then the
+ * one-wave ones, each costliest first. Output bytes and results never
+ * depend on the plan. Plans are per device and per format, process-wide;
+ * the default puts chunks costing at least 512 KiB on two waves. */
+#define CUDEC_MI355X_COST_BINS 256
+#define CUDEC_MI355X_FORMAT_LZ4 0
+#define CUDEC_MI355X_FORMAT_SNAPPY 1
+
+/* The cost bin of a chunk. */
+int cudec_mi355x_cost_bin(size_t compressed_size, size_t capacity);
+
+/* Sets or reads a format's plan: h_lanes_per_bin holds
+ * CUDEC_MI355X_COST_BINS entries, each 64 or 128 (anything else is
+ * CUDEC_ERR_INVALID_ARGUMENT, as are an unknown format and NULL). */
+cudec_status cudec_mi355x_set_decode_plan(int format, const uint16_t* h_lanes_per_bin);
+cudec_status cudec_mi355x_decode_plan(int format, uint16_t* h_lanes_per_bin);
+
+/* A format's plan by threshold: chunks whose cost is at least min_cost
+ * bytes decode on two waves, cheaper ones on one; 0 puts every chunk on
+ * two waves, SIZE_MAX none. The getters return the threshold last set
+ * (a plan set per bin since then is read with cudec_mi355x_decode_plan).
+ * The LZ4 setting applies to cudec_lz4_decompress_batch only, the Snappy
+ * one to cudec_snappy_decompress_batch only. */
 cudec_status cudec_mi355x_set_lz4_pipeline_threshold(size_t min_cost);
 cudec_status cudec_mi355x_lz4_pipeline_threshold(size_t* min_cost);
+cudec_status cudec_mi355x_set_snappy_pipeline_threshold(size_t min_cost);
+cudec_status cudec_mi355x_snappy_pipeline_threshold(size_t* min_cost);
 
-/* No counterpart in the reference: the autotuner of that threshold. It
+/* No counterpart in the reference: the autotuner of the LZ4 threshold. It
  * decodes the caller's batch (arguments as cudec_lz4_decompress_batch) once
  * per candidate threshold after a warmup, synchronously, keeps the fastest
  * for the current device and returns it in *h_best_min_cost. With them
@@ -64,6 +88,24 @@ cudec_status cudec_mi355x_autotune_lz4(const void* const* d_src_ptrs,
                                        cudec_chunk_result* d_results, cudec_stream_t stream,
                                        size_t* h_best_min_cost, size_t* h_candidates,
                                        float* h_ms);
+
+/* No counterpart in the reference: the per-bin autotuner of a format's
+ * plan, on the caller's batch (arguments as the format's batch entry),
+ * synchronously. It first sweeps the thresholds above, then, from the
+ * fastest threshold plan, visits each octave of cost the batch occupies,
+ * costliest first, and moves that octave's bins to the other lane count;
+ * a move is kept when the batch then decodes at least 1% faster (best of
+ * two event-timed launches after a warmup). The final plan is set for the
+ * current device and written to h_lanes_per_bin (CUDEC_MI355X_COST_BINS
+ * entries). With them non-NULL, *h_ms_threshold receives the best
+ * threshold plan's time and *h_ms_plan the final plan's, in milliseconds.
+ * The batch's outputs and results are those of the last run. */
+cudec_status cudec_mi355x_autotune_plan(int format, const void* const* d_src_ptrs,
+                                        const size_t* d_src_sizes, void* const* d_dst_ptrs,
+                                        const size_t* d_dst_capacities, size_t chunk_count,
+                                        cudec_chunk_result* d_results, cudec_stream_t stream,
+                                        uint16_t* h_lanes_per_bin, float* h_ms_threshold,
+                                        float* h_ms_plan);
 
 /* No counterpart in the reference: a 64-bit digest of each chunk's bytes,
  * computed on the device, for checking decoded output in place.

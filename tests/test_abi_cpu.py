@@ -300,3 +300,31 @@ def test_tuning_entries_reject_without_a_device_call():
     assert cudec_amd.autotune_lz4(None, 16, 16, 16, 1, 16, None)[0] == I
     assert cudec_amd.autotune_lz4(16, 16, 16, 16, 0, 16, None)[0] == I
     assert cudec_amd.lib().cudec_mi355x_lz4_pipeline_threshold(None) == I
+    assert cudec_amd.lib().cudec_mi355x_snappy_pipeline_threshold(None) == I
+    assert cudec_amd.autotune_plan("lz4", 16, 16, 16, 16, 1, 24, None)[0] == I
+    assert cudec_amd.autotune_plan("snappy", None, 16, 16, 16, 1, 16, None)[0] == I
+    assert cudec_amd.lib().cudec_mi355x_autotune_plan(7, 16, 16, 16, 16, 1, 16, None,
+                                                      16, None, None) == I
+    L = cudec_amd.lib()
+    ok = (ctypes.c_uint16 * cudec_amd.COST_BINS)(*([64] * cudec_amd.COST_BINS))
+    bad = (ctypes.c_uint16 * cudec_amd.COST_BINS)(*([64] * 255 + [96]))
+    assert L.cudec_mi355x_set_decode_plan(2, ok) == I          # unknown format
+    assert L.cudec_mi355x_set_decode_plan(0, None) == I
+    assert L.cudec_mi355x_set_decode_plan(0, bad) == I          # 96 lanes is no plan entry
+    assert L.cudec_mi355x_decode_plan(1, None) == I
+
+
+def test_cost_bins_follow_the_cost_order():
+    """cost = capacity + (capacity - compressed): four bins per octave,
+    bin 0 the costliest, and a costlier chunk never gets a larger bin."""
+    cb = cudec_amd.cost_bin
+    assert cb(0, 0) == 255 and cb(1, 0) == 255
+    assert cb(0, 1) == 253                       # cost 2
+    costs = sorted({c for k in range(2, 40) for c in (1 << k, (1 << k) + (1 << (k - 2)),
+                                                       (1 << k) + 3 * (1 << (k - 2)))})
+    bins = [cb(c, c) for c in costs]            # compressed == capacity: cost == capacity
+    assert bins == sorted(bins, reverse=True)
+    assert cb(1 << 20, 1 << 20) - cb(1 << 21, 1 << 21) == 4    # one octave = four bins
+    # matches cost time: a 4 MiB chunk that compresses to 256 KiB is costlier
+    # than a 4 MiB literal-only one
+    assert cb(256 << 10, 4 << 20) < cb(4 << 20, 4 << 20)
