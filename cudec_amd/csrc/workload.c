@@ -152,15 +152,32 @@ enum {
  * Draw order per step: one draw for the kind; literal: one draw for the
  * length, then one draw per four symbols (16 bits each); copy: one draw for
  * the length, one for the short/long choice, one for the offset. */
+/* What a token-process chunk drew (wl_gen_chunk_stats). */
+typedef struct {
+    uint64_t steps;          /* steps at pos > 0 (the first step is forced) */
+    uint64_t literal_runs;   /* literal steps at pos > 0 */
+    uint64_t literal_bytes;  /* bytes of all literal runs, before truncation */
+    uint64_t copies;
+    uint64_t copy_bytes;     /* before truncation */
+    uint64_t short_offsets;  /* copies whose offset is in [1, 16] */
+} wl_token_stats;
+
 static void gen_tokens(uint8_t* out, uint64_t size, uint64_t seed,
                        uint64_t p_lit_q64, uint32_t max_off,
-                       uint64_t p_short_q64, uint32_t short_max) {
+                       uint64_t p_short_q64, uint32_t short_max, wl_token_stats* st) {
     uint64_t s = seed;
     uint64_t pos = 0;
     while (pos < size) {
         const uint64_t kind = splitmix64(&s);
+        if (st != NULL && pos != 0) {
+            st->steps++;
+        }
         if (pos == 0 || kind < p_lit_q64) {
             uint64_t len = 1 + geometric(&g_geo12, splitmix64(&s));
+            if (st != NULL && pos != 0) {
+                st->literal_runs++;
+                st->literal_bytes += len;
+            }
             if (len > size - pos) {
                 len = size - pos;
             }
@@ -181,6 +198,11 @@ static void gen_tokens(uint8_t* out, uint64_t size, uint64_t seed,
                 window = pos;
             }
             const uint64_t off = 1 + below32(splitmix64(&s), (uint32_t)window);
+            if (st != NULL) {
+                st->copies++;
+                st->copy_bytes += len;
+                st->short_offsets += off <= 16 ? 1 : 0;
+            }
             if (len > size - pos) {
                 len = size - pos;
             }
@@ -212,19 +234,41 @@ static void gen_random(uint8_t* out, uint64_t size, uint64_t seed) {
     }
 }
 
-void wl_gen_chunk(uint8_t* out, uint64_t size, uint64_t seed, int32_t kind) {
+static void gen_chunk(uint8_t* out, uint64_t size, uint64_t seed, int32_t kind,
+                      wl_token_stats* st) {
     pthread_once(&g_tables_once, build_tables);
     switch (kind) {
         case WL_KIND_RANDOM:
             gen_random(out, size, seed);
             break;
         case WL_KIND_REPETITIVE:
-            gen_tokens(out, size, seed, Q64(5, 100), 8, 0, 8);
+            gen_tokens(out, size, seed, Q64(5, 100), 8, 0, 8, st);
             break;
         default:
-            gen_tokens(out, size, seed, Q64(45, 100), 65535, Q64(30, 100), 16);
+            gen_tokens(out, size, seed, Q64(45, 100), 65535, Q64(30, 100), 16, st);
             break;
     }
+}
+
+void wl_gen_chunk(uint8_t* out, uint64_t size, uint64_t seed, int32_t kind) {
+    gen_chunk(out, size, seed, kind, NULL);
+}
+
+/* wl_gen_chunk, also reporting the draws of a token-process or repetitive
+ * chunk into stats[0..5]: steps at pos > 0, literal runs among them,
+ * literal bytes, copies, copy bytes, copies at offsets in [1, 16]. The
+ * bytes are the same as wl_gen_chunk's. */
+void wl_gen_chunk_stats(uint8_t* out, uint64_t size, uint64_t seed, int32_t kind,
+                        uint64_t* stats) {
+    wl_token_stats st;
+    memset(&st, 0, sizeof(st));
+    gen_chunk(out, size, seed, kind, &st);
+    stats[0] = st.steps;
+    stats[1] = st.literal_runs;
+    stats[2] = st.literal_bytes;
+    stats[3] = st.copies;
+    stats[4] = st.copy_bytes;
+    stats[5] = st.short_offsets;
 }
 
 /* ------------------------------------------------------------- plans */

@@ -25,15 +25,47 @@ def oracle_decode(fmt: str, stream: bytes, cap: int):
     return st, out
 
 
+# Launch shapes of the parity tests: the plain form, and the tuned form with
+# every chunk on two pipelined waves (threshold 0) or every chunk on one wave
+# (threshold SIZE_MAX), so both of the tuned kernel's parsers and copiers
+# see every case. "tuned" alone runs the tuned form with the current plan.
+LAUNCHES = ("plain", "tuned-pipe", "tuned-split")
+_SHAPE_THRESHOLD = {"tuned-pipe": 0, "tuned-split": (1 << 64) - 1}
+
+
+class launch_shape:
+    """Sets the format's pipelining threshold for a launch shape and puts the
+    previous one back afterwards."""
+
+    def __init__(self, fmt: str, launch: str):
+        self.fmt, self.launch = fmt, launch
+        self.old = None
+
+    def __enter__(self):
+        import cudec_amd
+        if self.launch in _SHAPE_THRESHOLD:
+            st, self.old = cudec_amd.pipeline_threshold(self.fmt)
+            assert st == 0
+            assert cudec_amd.set_pipeline_threshold(self.fmt, _SHAPE_THRESHOLD[self.launch]) == 0
+        return "plain" if self.launch == "plain" else "tuned"
+
+    def __exit__(self, *exc):
+        import cudec_amd
+        if self.old is not None:
+            assert cudec_amd.set_pipeline_threshold(self.fmt, self.old) == 0
+        return False
+
+
 def run_device(fmt: str, form: str, streams, caps, poison=POISON, stream=None):
     import torch
 
     from cudec_amd.device import DeviceBatch
     db = DeviceBatch.from_streams(list(streams), caps, poison=poison)
     db.reset_results()
-    st = db.run(fmt, form, stream=stream)
-    assert st == 0, f"submission failed: {st}"
-    torch.cuda.synchronize()
+    with launch_shape(fmt, form) as f:
+        st = db.run(fmt, f, stream=stream)
+        assert st == 0, f"submission failed: {st}"
+        torch.cuda.synchronize()
     res = db.results()
     arena = db.output_arena()
     return db, res, arena

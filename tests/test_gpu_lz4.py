@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from tests import corpus, libs
-from tests.gpu_util import FORMS, POISON, check_against_oracle, gpu, run_device
+from tests.gpu_util import FORMS, LAUNCHES, POISON, check_against_oracle, gpu, run_device
 
 pytestmark = gpu
 
@@ -79,7 +79,7 @@ def test_contract_and_empty_blocks(form, hip):
     assert np.all(r["s"] == 0) and np.all(r["r"] == 0) and np.all(r["w"] == 0)
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_fixture_pairs_with_poison(form, lz4_fixtures):
     """tests/gpu_fixture.cu:172-186: capacity original + 8, poison beyond
     bytes_written must survive."""
@@ -91,7 +91,7 @@ def test_fixture_pairs_with_poison(form, lz4_fixtures):
         assert int(res["bytes_written"][i]) == len(f.original)
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_mutant_parity(form, lz4_fixtures):
     """tests/gpu_fixture.cu:188-230: the mutant corpus, held here to exact
     status and byte parity with the oracle, plus the reference's
@@ -139,7 +139,7 @@ def test_grid_stride_wraparound(form):
     assert np.all(r["s"] == 0) and np.all(r["w"] == 0)
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_adversarial_corpus_terminates(form):
     """tests/termination_gpu.cu: the hand-crafted hostile corpus at several
     capacities decodes to the oracle's verdict and terminates."""
@@ -152,7 +152,7 @@ def test_adversarial_corpus_terminates(form):
     check_against_oracle("lz4", form, streams, caps, names)
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_short_streams_exhaustive(form):
     """Every 1-byte stream, and every 2-byte stream with a second byte on a
     stride, at capacities 0, 1, 16 and 64: exact oracle parity."""
@@ -167,7 +167,7 @@ def test_short_streams_exhaustive(form):
     check_against_oracle("lz4", form, streams, caps)
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_last_literals_and_extension_sweeps(form):
     """The boundary families of tests/parser_twin.cpp:88-209 on the device."""
     streams, caps = [], []
@@ -195,20 +195,28 @@ def test_last_literals_and_extension_sweeps(form):
     check_against_oracle("lz4", form, streams, caps)
 
 
-@pytest.mark.parametrize("form", FORMS)
-def test_generated_workloads_exact(form):
-    """Seeded token-process and uneven chunks (all three kinds), compressed
-    by liblz4: every chunk decodes to its original."""
+@pytest.mark.parametrize("form", LAUNCHES)
+def test_generated_workloads_against_oracle(form):
+    """The benchmark configurations' chunk kinds at reduced count, every
+    chunk held to the oracle (status, bytes_written, bytes, untouched
+    poison): token-process chunks of exactly 64 KiB (the shape of the CPU
+    and plain-GPU configurations), and uneven chunks of all three kinds
+    (token process, uniform random, repetitive) from 1 KiB to 4 MiB, with
+    every seventh truncated by 1-9 bytes and every eleventh given a
+    capacity one byte short."""
     from cudec_amd import workload
-    b1 = workload.token64k(64, seed=0x77)
-    b2 = workload.uneven(120, seed=0x78, min_size=1024, octaves=8)
+    b1 = workload.token64k(48, seed=0x77)
+    b2 = workload.uneven(72, seed=0x78, min_size=1024, octaves=12)
     for b in (b1, b2):
         streams = [b.compressed(i) for i in range(b.n)]
         caps = [int(s) for s in b.sizes]
-        db, res, arena = run_device("lz4", form, streams, caps, poison=None)
-        assert np.all(res["status"] == 0)
-        assert np.array_equal(res["bytes_written"], b.sizes)
-        assert np.array_equal(arena[: b.total_uncompressed], b.orig)
+        for i in range(0, b.n, 7):
+            streams[i] = streams[i][: max(len(streams[i]) - 1 - i % 9, 0)]
+        for i in range(3, b.n, 11):
+            caps[i] = max(caps[i] - 1, 0)
+        res = check_against_oracle("lz4", form, streams, caps,
+                                   [f"{b.meta['workload']}#{i}" for i in range(b.n)])
+        assert np.sum(res["status"] == 0) > b.n // 2 and np.sum(res["status"] != 0) > 0
 
 
 def test_forms_and_streams_deterministic(lz4_fixtures):
@@ -262,7 +270,7 @@ def test_forms_and_streams_deterministic(lz4_fixtures):
     assert ok_slots(db, res, db.output_arena()) == ref_slots
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_long_periodic_matches(form):
     """Periodic data compresses to long matches (two or more extension
     bytes) at the period's offset, ending at the chunk's last byte: the

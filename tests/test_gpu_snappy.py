@@ -8,12 +8,12 @@ import numpy as np
 import pytest
 
 from tests import corpus, libs, snappy_vectors as sv
-from tests.gpu_util import FORMS, check_against_oracle, gpu, run_device
+from tests.gpu_util import FORMS, LAUNCHES, check_against_oracle, gpu, run_device
 
 pytestmark = gpu
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_fixtures_and_mutants(form, snappy_fixtures):
     streams, caps, names = [], [], []
     for f in snappy_fixtures:
@@ -33,7 +33,7 @@ def test_fixtures_and_mutants(form, snappy_fixtures):
                 assert ok, names[i]
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_crafted_capacity_and_accept_sets(form):
     streams, caps, names = [], [], []
     for c in sv.crafted_negatives():
@@ -62,7 +62,7 @@ def test_crafted_capacity_and_accept_sets(form):
     assert np.all(res["status"][:n_neg] == libs.CORRUPT)
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_short_streams_exhaustive(form):
     streams, caps = [b""], [0]
     for cap in (0, 1, 64):
@@ -75,7 +75,7 @@ def test_short_streams_exhaustive(form):
     check_against_oracle("snappy", form, streams, caps)
 
 
-@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("form", LAUNCHES)
 def test_generated_batches_with_malformed_chunks(form):
     """Seeded uneven chunks through snappy_compress; a seeded 2% are
     truncated or have one tag-stream byte flipped; the malformed ones get
