@@ -48,7 +48,8 @@ ABI_SYMBOLS = {
         "cudec_mi355x_set_decode_plan", "cudec_mi355x_decode_plan",
         "cudec_mi355x_set_lz4_pipeline_threshold", "cudec_mi355x_lz4_pipeline_threshold",
         "cudec_mi355x_set_snappy_pipeline_threshold", "cudec_mi355x_snappy_pipeline_threshold",
-        "cudec_mi355x_autotune_lz4", "cudec_mi355x_autotune_plan", "cudec_mi355x_digest_batch",
+        "cudec_mi355x_autotune_lz4", "cudec_mi355x_autotune_plan",
+        "cudec_mi355x_lz4_decompress_batch_noalloc", "cudec_mi355x_digest_batch",
         "cudec_mi355x_serve_lz4",
     ],
 }
@@ -81,7 +82,8 @@ def lib() -> ctypes.CDLL:
         L.cudec_version.restype = ctypes.c_int
         for name in ("cudec_lz4_decompress_batch", "cudec_snappy_decompress_batch",
                      "cudec_lz4_decompress_batch_plain",
-                     "cudec_snappy_decompress_batch_plain"):
+                     "cudec_snappy_decompress_batch_plain",
+                     "cudec_mi355x_lz4_decompress_batch_noalloc"):
             fn = getattr(L, name)
             fn.argtypes = _BATCH_ARGS
             fn.restype = ctypes.c_int

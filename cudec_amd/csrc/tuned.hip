@@ -46,6 +46,7 @@
 #include <mutex>
 
 #include "common.h"
+#include "cudec_mi355x.h"
 #include "ladder.h"
 
 namespace cudec_amd {
@@ -2115,10 +2116,23 @@ struct DeviceState {
     int split_blocks_per_cu = 0; /* piped_batch<_, 5>: no two-wave chunk, or Snappy */
     Plan plan[2];                /* per format (kFormatLz4, kFormatSnappy) */
     uint64_t min_cost[2];        /* the threshold each plan was last set from */
-    hipMemPool_t pool = nullptr;
+    hipMemPool_t pool = nullptr; /* made on the first allocating launch */
+    bool pool_tried = false;
+    void* fixed = nullptr;       /* this device's address of fixed_scratch */
+    hipError_t fixed_status = hipErrorNotReady;
     bool ready = false;
 };
 constexpr int kMaxDevices = 64;
+
+/* Scratch of the allocation-free launch (launch_tuned_fixed): the queue and
+ * the order of up to kFixedMaxChunks chunks (4.04 MiB), one copy per device
+ * for the life of the process. Host code never passes this name to a
+ * kernel or a copy: on the host it is a stand-in, and each device's
+ * address of its own copy comes from hipGetSymbolAddress
+ * (DeviceState::fixed). */
+constexpr size_t kFixedMaxChunks = CUDEC_MI355X_NOALLOC_MAX_CHUNKS;
+constexpr size_t kFixedBytes = sizeof(Queue) + sizeof(uint32_t) * kFixedMaxChunks;
+static __device__ __align__(256) uint8_t scratch[kFixedBytes];
 
 /* The plan that puts chunks costing at least min_cost on two waves: every
  * bin up to the bin of a chunk whose cost is exactly min_cost (costlier
@@ -2169,13 +2183,34 @@ hipError_t device_state(DeviceState** out) {
             d.plan[f] = plan_from_threshold(kDefaultPipeMinCost);
             d.min_cost[f] = kDefaultPipeMinCost;
         }
+        /* this device is current: its own copy of the fixed scratch */
+        d.fixed_status = hipGetSymbolAddress(&d.fixed, HIP_SYMBOL(scratch));
+        if (d.fixed_status != hipSuccess) {
+            d.fixed = nullptr;
+            (void)hipGetLastError(); /* only the allocation-free entry needs it */
+        }
+        d.ready = true;
+    }
+    *out = &d;
+    return hipSuccess;
+}
+
+/* The device's scratch pool, made on first use. It keeps freed scratch up
+ * to kPoolKeep bytes, so that a stream of launches does not return memory
+ * to the driver between them, while one very large batch's scratch is
+ * given back once it is freed. */
+constexpr uint64_t kPoolKeep = 64ull << 20;
+hipMemPool_t scratch_pool(DeviceState& d, int dev) {
+    std::lock_guard<std::mutex> hold(g_devices_lock);
+    if (!d.pool_tried) {
+        d.pool_tried = true;
         hipMemPoolProps props = {};
         props.allocType = hipMemAllocationTypePinned;
         props.location.type = hipMemLocationTypeDevice;
         props.location.id = dev;
         hipMemPool_t pool = nullptr;
         if (hipMemPoolCreate(&pool, &props) == hipSuccess) {
-            uint64_t keep = UINT64_MAX;
+            uint64_t keep = kPoolKeep;
             if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) {
                 d.pool = pool;
             } else {
@@ -2183,10 +2218,8 @@ hipError_t device_state(DeviceState** out) {
             }
         }
         (void)hipGetLastError(); /* a pool that could not be made is not an error */
-        d.ready = true;
     }
-    *out = &d;
-    return hipSuccess;
+    return d.pool;
 }
 
 }  // namespace
@@ -2256,40 +2289,20 @@ uint32_t host_cost_bin(uint64_t src_size, uint64_t cap) {
     return kBins - 1 - cost_key(cost);
 }
 
-hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_sizes,
-                        void* const* dsts, const size_t* caps, size_t n,
-                        cudec_chunk_result* results, hipStream_t stream) {
-    if (n > UINT32_MAX) {
-        /* the queue indexes chunks in 32 bits; larger batches take the
-         * plain form, which has no such limit */
-        return launch_plain(format, srcs, src_sizes, dsts, caps, n, results, stream);
-    }
-    DeviceState* dev = nullptr;
-    hipError_t e = device_state(&dev);
-    if (e != hipSuccess) {
-        return e;
-    }
+namespace {
+
+/* Bins the batch into the queue held by `work` (a Queue, then order[n])
+ * and launches the persistent decoder, all on `stream`. */
+hipError_t queue_and_decode(DeviceState* dev, int format, const void* const* srcs,
+                            const size_t* src_sizes, void* const* dsts, const size_t* caps,
+                            size_t n, cudec_chunk_result* results, void* work,
+                            hipStream_t stream) {
     Plan plan;
     {
         std::lock_guard<std::mutex> hold(g_devices_lock);
         plan = dev->plan[format == kFormatSnappy ? kFormatSnappy : kFormatLz4];
     }
-    const size_t bytes = sizeof(Queue) + sizeof(uint32_t) * n;
-    /* under stream capture the scratch becomes a graph allocation node of
-     * the device's default pool; eager launches use the keeping pool */
-    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
-    e = hipStreamIsCapturing(stream, &capture);
-    if (e != hipSuccess) {
-        return e;
-    }
-    void* scratch = nullptr;
-    e = dev->pool != nullptr && capture == hipStreamCaptureStatusNone
-            ? hipMallocFromPoolAsync(&scratch, bytes, dev->pool, stream)
-            : hipMallocAsync(&scratch, bytes, stream);
-    if (e != hipSuccess) {
-        return e;
-    }
-    Queue* q = static_cast<Queue*>(scratch);
+    Queue* q = static_cast<Queue*>(work);
     uint32_t* order = reinterpret_cast<uint32_t*>(q + 1);
     const unsigned tb = 256;
     const unsigned nb = (unsigned)((n + tb - 1) / tb);
@@ -2303,9 +2316,9 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
         any_two_wave = any_two_wave || plan.two_wave[w] != 0;
     }
     /* two-wave blocks; a block splits into two one-wave decoders when it
-     * meets a chunk below the threshold. The five-wave build runs where no
-     * chunk is two-wave (every block splits at once) and for Snappy, whose
-     * two-wave path fits its 96 VGPRs. */
+     * meets a one-wave chunk. The five-wave build runs where no chunk is
+     * two-wave (every block splits at once) and for Snappy, whose two-wave
+     * path fits its 96 VGPRs. */
     const bool five_waves = !any_two_wave || format == kFormatSnappy;
     size_t grid = (size_t)dev->cus *
                   (size_t)(five_waves ? dev->split_blocks_per_cu : dev->pipe_blocks_per_cu);
@@ -2331,9 +2344,66 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
         default: /* Snappy always takes the five-wave build */
             return hipErrorInvalidValue;
     }
-    e = hipGetLastError();
-    const hipError_t f = hipFreeAsync(scratch, stream);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_sizes,
+                        void* const* dsts, const size_t* caps, size_t n,
+                        cudec_chunk_result* results, hipStream_t stream) {
+    if (n > UINT32_MAX) {
+        /* the queue indexes chunks in 32 bits; larger batches take the
+         * plain form, which has no such limit */
+        return launch_plain(format, srcs, src_sizes, dsts, caps, n, results, stream);
+    }
+    DeviceState* dev = nullptr;
+    hipError_t e = device_state(&dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    int id = 0;
+    e = hipGetDevice(&id);
+    if (e != hipSuccess) {
+        return e;
+    }
+    hipMemPool_t pool = scratch_pool(*dev, id);
+    const size_t bytes = sizeof(Queue) + sizeof(uint32_t) * n;
+    /* under stream capture the scratch becomes a graph allocation node of
+     * the device's default pool; eager launches use the keeping pool */
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    e = hipStreamIsCapturing(stream, &capture);
+    if (e != hipSuccess) {
+        return e;
+    }
+    void* mem = nullptr;
+    e = pool != nullptr && capture == hipStreamCaptureStatusNone
+            ? hipMallocFromPoolAsync(&mem, bytes, pool, stream)
+            : hipMallocAsync(&mem, bytes, stream);
+    if (e != hipSuccess) {
+        return e;
+    }
+    e = queue_and_decode(dev, format, srcs, src_sizes, dsts, caps, n, results, mem, stream);
+    const hipError_t f = hipFreeAsync(mem, stream);
     return e != hipSuccess ? e : f;
+}
+
+hipError_t launch_tuned_fixed(int format, const void* const* srcs, const size_t* src_sizes,
+                              void* const* dsts, const size_t* caps, size_t n,
+                              cudec_chunk_result* results, hipStream_t stream) {
+    if (n > kFixedMaxChunks) {
+        return hipErrorInvalidValue; /* the entry point refuses this first */
+    }
+    DeviceState* dev = nullptr;
+    hipError_t e = device_state(&dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    if (dev->fixed == nullptr) {
+        return dev->fixed_status != hipSuccess ? dev->fixed_status : hipErrorInvalidSymbol;
+    }
+    return queue_and_decode(dev, format, srcs, src_sizes, dsts, caps, n, results, dev->fixed,
+                            stream);
 }
 
 }  // namespace cudec_amd

@@ -107,6 +107,27 @@ cudec_status cudec_mi355x_autotune_plan(int format, const void* const* d_src_ptr
                                         uint16_t* h_lanes_per_bin, float* h_ms_threshold,
                                         float* h_ms_plan);
 
+/* No counterpart in the reference. cudec_lz4_decompress_batch for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc. Same arguments, contract,
+ * statuses, bytes and results as cudec_lz4_decompress_batch (it runs the
+ * same kernels), but the launch's scratch (its work queue) is a fixed
+ * device array inside the library, about 4 MiB on each GPU it runs on, held
+ * until the process ends, instead of a stream-ordered allocation. It calls
+ * no allocating runtime routine. At most CUDEC_MI355X_NOALLOC_MAX_CHUNKS
+ * chunks per call: more is CUDEC_ERR_INVALID_ARGUMENT, with no device call.
+ * Every call on one GPU uses that GPU's one array: calls whose launches
+ * overlap in time on one GPU (different streams, other host threads) race
+ * and may decode wrongly, so order them (one stream, or events). */
+#define CUDEC_MI355X_NOALLOC_MAX_CHUNKS 1048576
+cudec_status cudec_mi355x_lz4_decompress_batch_noalloc(const void* const* d_src_ptrs,
+                                                       const size_t* d_src_sizes,
+                                                       void* const* d_dst_ptrs,
+                                                       const size_t* d_dst_capacities,
+                                                       size_t chunk_count,
+                                                       cudec_chunk_result* d_results,
+                                                       cudec_stream_t stream);
+
 /* No counterpart in the reference: a 64-bit digest of each chunk's bytes,
  * computed on the device, for checking decoded output in place.
  * Chunk i covers d_ptrs[i][0, len_i), where len_i is d_results[i]

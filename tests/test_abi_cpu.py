@@ -328,3 +328,14 @@ def test_cost_bins_follow_the_cost_order():
     # matches cost time: a 4 MiB chunk that compresses to 256 KiB is costlier
     # than a 4 MiB literal-only one
     assert cb(256 << 10, 4 << 20) < cb(4 << 20, 4 << 20)
+
+
+def test_noalloc_rejects_without_a_device_call():
+    """The allocation-free entry takes the batch entry's argument checks and
+    refuses a batch above its fixed scratch, all before any device call."""
+    fn = cudec_amd.lib().cudec_mi355x_lz4_decompress_batch_noalloc
+    I = cudec_amd.CUDEC_ERR_INVALID_ARGUMENT
+    assert fn(16, 16, 16, 16, (1 << 20) + 1, 16, None) == I
+    assert fn(16, 16, 16, 16, 0, 16, None) == I
+    assert fn(None, 16, 16, 16, 1, 16, None) == I
+    assert fn(16, 16, 16, 16, 1, 24, None) == I
