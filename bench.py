@@ -39,7 +39,7 @@ def log(msg: str) -> None:
 def parse_args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--steps", type=int, default=30)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", choices=["uneven", "token64k"], default="uneven")
     p.add_argument("--format", choices=["lz4", "snappy"], default="lz4")
@@ -47,6 +47,7 @@ def parse_args():
     p.add_argument("--seed", type=int, default=0x5EED)
     p.add_argument("--forms", default="tuned,plain")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-parse", action="store_true", help="skip the parse-only ceiling leg")
     p.add_argument("--cpu-threads", type=int, default=None)
     p.add_argument("--traffic-file", default=None,
                    help="JSON with measured HBM bytes per launch (rocprofv3 PMC)")
@@ -60,10 +61,10 @@ def parse_args():
     p.add_argument("--no-graph", action="store_true",
                    help="skip the HIP-graph-captured replay of the tuned launch")
     p.add_argument("--no-autotune", action="store_true",
-                   help="keep the default LZ4 pipelining threshold instead of tuning it "
+                   help="keep the default decode plan instead of tuning it "
                         "on the batch before timing")
     p.add_argument("--threshold", type=int, default=None,
-                   help="set the LZ4 pipelining threshold (bytes of cost; -1: none) instead of "
+                   help="set the format's pipelining threshold (bytes of cost; -1: none) instead of "
                         "tuning it, so a profiled run repeats one launch shape")
     p.add_argument("--serve-rates", default="2000,8000,32000",
                    help="request rates (per second) of the serving leg; empty to skip it")
@@ -173,6 +174,39 @@ def time_form(db, fmt, form, steps, warmup, stream, orig_dev, batch, valid):
     return wall, kernel_ms, res
 
 
+def parse_leg(db, fmt, steps, warmup, stream, eager):
+    """The parse ceiling: the tuned launch's parse-only build
+    (cudec_mi355x_parse_batch) on the same batch and plan, event-timed over
+    `steps` launches after `warmup`; its statuses and sizes must equal the
+    decode's."""
+    import numpy as np
+    import torch
+    import cudec_amd
+    from cudec_amd.serve import percentile
+
+    def launch():
+        return cudec_amd.lib().cudec_mi355x_parse_batch(
+            cudec_amd.FORMAT_CODE[fmt], db.src_ptrs.data_ptr(), db.src_sizes.data_ptr(),
+            db.dst_ptrs.data_ptr(), db.dst_caps.data_ptr(), db.n, db.d_results.data_ptr(),
+            stream.cuda_stream)
+    for _ in range(warmup):
+        assert launch() == 0
+    torch.cuda.synchronize()
+    if not np.array_equal(db.results(), eager):
+        raise SystemExit("[bench] parse-only results differ from the decode's")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for k in range(steps):
+        starts[k].record(stream)
+        assert launch() == 0
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
+    return {"avg_kernel_ms": round(sum(ms) / len(ms), 4),
+            "p50_kernel_ms": round(percentile(ms, 50), 4),
+            "launches": steps, "results_equal_decode": True}
+
+
 def graph_leg(db, fmt, steps, stream, orig_dev, batch, valid, eager):
     """The tuned launch captured once into a HIP graph (cudec_amd/graph.py)
     and replayed `steps` times, event-timed; the output is cleared before
@@ -210,112 +244,177 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(batch, fmt, threads, valid, runs=30):
-    """The reference ladder compiled from its own source (oracle/_ref) when
-    that build is present, else the C oracle: `threads` host threads, one
-    pass over the whole batch into pre-sized buffers, decode calls only.
-    Returns the baseline record and every chunk's (status, bytes_written),
-    which the GPU results are held to."""
-    import numpy as np
+# The box's CPU share for one GPU is 16 threads; the harness never runs more
+# worker threads than that, whatever the host shows.
+CPU_SHARE = 16
+
+DEC_LIBLZ4, DEC_LIBSNAPPY, DEC_TWIN_LZ4, DEC_TWIN_SNAPPY = 0, 1, 2, 3
+
+
+def cpu_threads(requested=None) -> tuple[int, int]:
+    """(threads to use, cores this process may run on)."""
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        (os.cpu_count() or 1)
+    return (requested or max(1, min(visible, CPU_SHARE))), visible
+
+
+def twin_decoders(fmt):
+    """The reference's host ladders compiled from its source (oracle/_ref,
+    kind "reference") where that build is present, else the oracle's C
+    restatement (kind "port"): (function address, kind, name)."""
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libcudec_ref.so")
     port_path = os.path.join(ROOT, "oracle", "liboracle.so")
-    u64 = ctypes.c_uint64
     if os.path.exists(ref_path):
-        lib = ctypes.CDLL(ref_path)
-        fn = lib.ref_lz4_decode if fmt == "lz4" else lib.ref_snappy_decode
-        kind = "reference"
+        lib, kind, prefix = ctypes.CDLL(ref_path), "reference", "ref"
     else:
-        lib = ctypes.CDLL(port_path)
-        fn = lib.oracle_lz4_decode if fmt == "lz4" else lib.oracle_snappy_decode
-        kind = "port"
-    extra = [] if fmt == "lz4" else [ctypes.c_void_p]
-    fn.argtypes = [ctypes.c_void_p, u64, ctypes.c_void_p, u64, ctypes.POINTER(u64)] + extra
-    fn.restype = ctypes.c_int32
-    out = np.empty(batch.total_uncompressed, dtype=np.uint8)
-    status = np.zeros(batch.n, dtype=np.int32)
-    written = np.zeros(batch.n, dtype=np.uint64)
-    cbase = batch.comp.ctypes.data
-    obase = out.ctypes.data
-    n = batch.n
-    bounds = [n * t // threads for t in range(threads + 1)]
+        lib, kind, prefix = ctypes.CDLL(port_path), "port", "oracle"
+    fn = getattr(lib, f"{prefix}_{fmt}_decode")
+    return ctypes.cast(fn, ctypes.c_void_p).value, kind, f"{prefix}_{fmt}_decode", lib
 
-    def work(t):
-        w = u64(0)
-        for i in range(bounds[t], bounds[t + 1]):
-            args = [cbase + int(batch.comp_off[i]), int(batch.comp_size[i]),
-                    obase + int(batch.orig_off[i]), int(batch.sizes[i]), ctypes.byref(w)]
-            if extra:
-                args.append(None)
-            status[i] = fn(*args)
-            written[i] = w.value if status[i] == 0 else 0
-        return 0
 
-    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-        t0 = time.perf_counter()
-        list(ex.map(work, range(threads)))
-        first = time.perf_counter() - t0
-        intact_ok = bool(np.all(status[valid] == 0)) and \
-            bool(np.array_equal(written[valid], batch.sizes[valid]))
-        if intact_ok and bool(np.all(valid)):
-            intact_ok = bool(np.array_equal(out, batch.orig))
-        elif intact_ok:
-            for i in np.nonzero(valid)[0]:
-                o, m = int(batch.orig_off[i]), int(batch.sizes[i])
-                if not np.array_equal(out[o:o + m], batch.orig[o:o + m]):
-                    intact_ok = False
-                    break
-        if not intact_ok:
-            raise SystemExit("[bench] CPU baseline produced wrong output")
-        # the checked pass is the warmup; then timed passes, bounded to about
-        # 10 s of wall time
-        runs = max(1, min(runs, int(10.0 / max(first, 1e-3))))
-        secs = []
-        for _ in range(runs):
-            t0 = time.perf_counter()
-            list(ex.map(work, range(threads)))
-            secs.append(time.perf_counter() - t0)
+class CpuBatch:
+    """Host pointer tables of a batch, and output buffers sized to it."""
+
+    def __init__(self, batch, m=None):
+        import numpy as np
+        m = batch.n if m is None else m
+        self.n = m
+        self.out = np.empty(max(int(batch.sizes[:m].sum()), 1), dtype=np.uint8)
+        self.srcs = (batch.comp_off[:m] + np.uint64(batch.comp.ctypes.data)).astype(np.uint64)
+        self.sizes = batch.comp_size[:m].astype(np.uint64)
+        off = batch.orig_off[:m] - batch.orig_off[0] if m else batch.orig_off[:0]
+        self.dsts = (off + np.uint64(self.out.ctypes.data)).astype(np.uint64)
+        self.caps = batch.sizes[:m].astype(np.uint64)
+        self.bytes = int(batch.sizes[:m].sum())
+        self.algo = self.bytes + int(batch.comp_size[:m].sum())
+
+    def time(self, kind, fn, threads, passes):
+        """Per-pass seconds, and (status, written) of the last pass."""
+        import numpy as np
+        from cudec_amd import workload
+        W = workload.wl()
+        W.wl_time_decode.argtypes = [ctypes.c_int32, ctypes.c_void_p] + [ctypes.c_void_p] * 4 + \
+            [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 3
+        W.wl_time_decode.restype = ctypes.c_int32
+        secs = np.zeros(passes, dtype=np.float64)
+        status = np.zeros(self.n, dtype=np.int32)
+        written = np.zeros(self.n, dtype=np.uint64)
+        ran = W.wl_time_decode(kind, fn, self.srcs.ctypes.data, self.sizes.ctypes.data,
+                               self.dsts.ctypes.data, self.caps.ctypes.data, self.n, threads,
+                               passes, secs.ctypes.data, status.ctypes.data, written.ctypes.data)
+        if ran < 0:
+            return None, None, None
+        return secs, status, written
+
+
+def cpu_row(cb, kind, fn, threads, name, warmup=3, runs=30, budget_s=10.0):
+    """warmup + up to `runs` timed passes (fewer when a pass is long, so the
+    row takes about budget_s), nearest-rank p50/p90/p99 of the pass times."""
     from cudec_amd.serve import percentile
+    first, status, written = cb.time(kind, fn, threads, 1)
+    if first is None:
+        return None, None, None
+    runs = max(3, min(runs, int(budget_s / max(float(first[0]), 1e-4))))
+    secs, _, _ = cb.time(kind, fn, threads, warmup + runs)
+    secs = secs[warmup:]
+    rate = lambda t: round(cb.bytes / t / 1e9, 3)
+    row = {"decoder": name, "threads": threads, "chunks": cb.n,
+           "uncompressed_bytes": cb.bytes,
+           "p50": rate(percentile(secs, 50)), "at_p90_time": rate(percentile(secs, 90)),
+           "at_p99_time": rate(percentile(secs, 99)),
+           "algo_gbps_p50": round(cb.algo / percentile(secs, 50) / 1e9, 3),
+           "passes": f"{warmup} warmup + {len(secs)} timed"}
+    return row, status, written
 
-    def rate(t, nbytes):
-        return round(nbytes / t / 1e9, 3)
-    # one thread over a sample: the batch's first chunks (the batch order is
-    # shuffled) up to 256 MiB of output, 3 warmup + 30 timed passes
+
+def cpu_baseline(batch, fmt, threads, valid, visible):
+    """The CPU decoders timed on the host's cores in this run: the
+    reference's host twin (oracle/_ref) and the compression library's own
+    decoder (liblz4 LZ4_decompress_safe, snappy_uncompress), on `threads`
+    threads (static contiguous partition) and on one thread, decode calls
+    only into pre-sized buffers (a C harness, cudec_amd/csrc/workload.c).
+    The twin's statuses and sizes over the whole batch are returned for the
+    GPU results to be held to; the twin's bytes are checked against the
+    originals."""
+    import numpy as np
+    from cudec_amd import workload
+    fn, kind, twin_name, _keep = twin_decoders(fmt)
+    twin = DEC_TWIN_LZ4 if fmt == "lz4" else DEC_TWIN_SNAPPY
+    lib_kind = DEC_LIBLZ4 if fmt == "lz4" else DEC_LIBSNAPPY
+    codec = workload.CODEC_LZ4 if fmt == "lz4" else workload.CODEC_SNAPPY
+    lib_name = ("LZ4_decompress_safe" if fmt == "lz4" else "snappy_uncompress") + \
+        f" ({workload.codec_version(codec)})"
+    full = CpuBatch(batch)
+    rows = []
+    row, status, written = cpu_row(full, twin, fn, threads, f"twin {twin_name} [{kind}]")
+    if row is None:
+        raise SystemExit("[bench] CPU baseline: the twin decoder could not run")
+    ok = bool(np.all(status[valid] == 0)) and \
+        bool(np.array_equal(written[valid], batch.sizes[valid]))
+    if ok:
+        for i in np.nonzero(valid)[0]:
+            o = int(batch.orig_off[i] - batch.orig_off[0])
+            n = int(batch.sizes[i])
+            if not np.array_equal(full.out[o:o + n], batch.orig[int(batch.orig_off[i]):
+                                                               int(batch.orig_off[i]) + n]):
+                ok = False
+                break
+    if not ok:
+        raise SystemExit("[bench] CPU baseline produced wrong output")
+    rows.append(row)
+    lib_row, lib_status, lib_written = cpu_row(full, lib_kind, None, threads, lib_name)
+    if lib_row is not None:
+        if not (np.all(lib_status[valid] == 0) and
+                np.array_equal(lib_written[valid], batch.sizes[valid])):
+            raise SystemExit("[bench] CPU baseline: the library decoder disagrees")
+        rows.append(lib_row)
+    # one thread over a sample: the batch's first chunks (the order is
+    # shuffled) up to 256 MiB of output
     m = int(np.searchsorted(np.cumsum(batch.sizes.astype(np.int64)), 256 << 20, side="right"))
-    m = max(1, min(m, n))
-    sample_bytes = int(batch.sizes[:m].sum())
-    single_bounds = (0, m)
-
-    def single():
-        w = u64(0)
-        for i in range(*single_bounds):
-            args = [cbase + int(batch.comp_off[i]), int(batch.comp_size[i]),
-                    obase + int(batch.orig_off[i]), int(batch.sizes[i]), ctypes.byref(w)]
-            if extra:
-                args.append(None)
-            fn(*args)
-
-    one = []
-    for k in range(33):
-        t0 = time.perf_counter()
-        single()
-        if k >= 3:
-            one.append(time.perf_counter() - t0)
-    total = batch.total_uncompressed
-    # rates at the nearest-rank p50 / p90 / p99 pass times
-    rec = {"value": rate(percentile(secs, 50), total), "unit": "GB/s (uncompressed)",
-           "cores": threads, "kind": kind,
-           "sample": f"the full batch ({n} chunks, {batch.total_uncompressed / 2**30:.2f} GiB "
-                     f"out), {threads} threads, static contiguous partition; 1 checked warmup "
-                     f"pass + {len(secs)} timed passes, nearest-rank p50 "
-                     f"({min(secs):.2f}-{max(secs):.2f} s wall per pass)",
-           "at_p90_time": rate(percentile(secs, 90), total),
-           "at_p99_time": rate(percentile(secs, 99), total),
-           "single_thread": {"value": rate(percentile(one, 50), sample_bytes),
-                             "at_p90_time": rate(percentile(one, 90), sample_bytes),
-                             "sample": f"first {m} chunks ({sample_bytes / 2**20:.0f} MiB out), "
-                                       "1 thread, 3 warmup + 30 timed passes, nearest-rank p50"},
-           "cpu_model": cpu_model()}
+    sample = CpuBatch(batch, max(1, min(m, batch.n)))
+    for k, f, name in ((twin, fn, f"twin {twin_name} [{kind}]"), (lib_kind, None, lib_name)):
+        r, _, _ = cpu_row(sample, k, f, 1, name)
+        if r is not None:
+            rows.append(r)
+    rec = {"value": rows[0]["p50"], "unit": "GB/s (uncompressed)", "cores": threads,
+           "kind": kind,
+           "sample": f"the full batch ({batch.n} chunks, {batch.total_uncompressed / 2**30:.2f} "
+                     f"GiB out), {threads} threads (static contiguous partition) of the "
+                     f"{visible} this process may run on (one GPU's share is {CPU_SHARE}), "
+                     f"decode calls only into pre-sized buffers; {rows[0]['passes']} passes, "
+                     "nearest-rank p50 of the pass times; single-thread rows decode the "
+                     f"batch's first {sample.n} chunks ({sample.bytes / 2**20:.0f} MiB out)",
+           "at_p90_time": rows[0]["at_p90_time"], "at_p99_time": rows[0]["at_p99_time"],
+           "cores_visible": visible, "cpu_model": cpu_model(), "rows": rows}
     return rec, status, written
+
+
+def cpu_config_token64k(threads, seed):
+    """The CPU configuration: 1024 chunks of exactly 64 KiB from the token
+    process, LZ4_compress_default; the twin and liblz4, one thread and
+    `threads` threads, 3 warmup + 30 timed passes each, every chunk checked
+    byte-exact first."""
+    import numpy as np
+    from cudec_amd import workload
+    b = workload.token64k(1024, seed=seed)
+    fn, kind, twin_name, _keep = twin_decoders("lz4")
+    cb = CpuBatch(b)
+    rows = []
+    for k, f, name in ((DEC_TWIN_LZ4, fn, f"twin {twin_name} [{kind}]"),
+                       (DEC_LIBLZ4, None,
+                        f"LZ4_decompress_safe ({workload.codec_version(workload.CODEC_LZ4)})")):
+        for t in (1, threads):
+            cb.out.fill(0)
+            r, status, written = cpu_row(cb, k, f, t, name, budget_s=30.0)
+            if r is None:
+                continue
+            if not (np.all(status == 0) and np.array_equal(written, b.sizes) and
+                    np.array_equal(cb.out, b.orig)):
+                raise SystemExit(f"[bench] CPU configuration: {name} output differs")
+            rows.append(r)
+    return {"workload": "1024 x 64 KiB token-process chunks, LZ4_compress_default "
+                        f"(ratio {b.total_uncompressed / b.total_compressed:.2f})",
+            "rows": rows}
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "lz4_uneven_hbm_traffic.md")
@@ -437,28 +536,36 @@ def main():
     if args.threshold is not None:
         import cudec_amd
         t = (1 << 64) - 1 if args.threshold < 0 else args.threshold
-        if cudec_amd.set_lz4_pipeline_threshold(t) != 0:
+        if cudec_amd.set_pipeline_threshold(args.format, t) != 0:
             raise SystemExit("[bench] could not set the pipelining threshold")
         log(f"pipelining threshold set to {t}")
-    elif args.format == "lz4" and "tuned" in forms and not args.no_autotune:
+    elif "tuned" in forms and not args.no_autotune:
         import cudec_amd
         t0 = time.perf_counter()
-        st, best, sweep = cudec_amd.autotune_lz4(
-            db.src_ptrs.data_ptr(), db.src_sizes.data_ptr(), db.dst_ptrs.data_ptr(),
+        st, lanes, ms_threshold, ms_plan = cudec_amd.autotune_plan(
+            args.format, db.src_ptrs.data_ptr(), db.src_sizes.data_ptr(), db.dst_ptrs.data_ptr(),
             db.dst_caps.data_ptr(), db.n, db.d_results.data_ptr(), stream.cuda_stream)
         if st != 0:
             raise SystemExit(f"[bench] autotune failed: {st}")
-        tuning = {"knob": "LZ4 chunks with cost (2 x capacity - compressed) >= threshold decode "
-                          "on two pipelined waves, cheaper ones on one",
-                  "best_threshold_bytes": best,
-                  "sweep_ms": {("all" if c == 0 else "none" if c >= 2**63 else str(c)):
-                               round(ms, 3) for c, ms in sweep}}
-        log(f"autotune ({time.perf_counter() - t0:.1f}s): best threshold {best}; {tuning['sweep_ms']}")
+        bins = sorted({cudec_amd.cost_bin(int(c), int(u))
+                       for c, u in zip(batch.comp_size, batch.sizes)})
+        tuning = {"knob": "lanes per chunk for each cost bin (cost = 2 x capacity - "
+                          "compressed): 128 = two pipelined waves, 64 = one wave; a threshold "
+                          "sweep, then each occupied octave moved to the other lane count when "
+                          "that gains 1%",
+                  "plan": {str(b): lanes[b] for b in bins},
+                  "threshold_plan_ms": round(ms_threshold, 3), "plan_ms": round(ms_plan, 3),
+                  "tuning_s": round(time.perf_counter() - t0, 2)}
+        log(f"autotune: {tuning}")
     for form in forms:
         wall, kms, res = time_form(db, args.format, form, args.steps, args.warmup, stream,
                                    orig_dev, batch, valid)
         avg_ms = sum(kms) / len(kms)
+        from cudec_amd.serve import percentile
         measured[form] = {"wall_s": wall, "kernel_ms": kms, "avg_kernel_ms": avg_ms,
+                          "p50_kernel_ms": percentile(kms, 50),
+                          "p90_kernel_ms": percentile(kms, 90),
+                          "p99_kernel_ms": percentile(kms, 99),
                           "results": res,
                           "gbps": uncomp * args.steps / wall / 1e9,
                           "kernel_gbps": uncomp / (avg_ms / 1e3) / 1e9,
@@ -467,17 +574,30 @@ def main():
             f"uncompressed (wall), {measured[form]['algo_gbps']:.1f} GB/s algorithmic "
             f"= {100 * measured[form]['algo_gbps'] / HBM_PEAK_GBPS:.1f}% of HBM peak")
     head = measured[forms[0]]
+    parse = None
+    if "tuned" in measured and not args.no_parse:
+        parse = parse_leg(db, args.format, args.steps, args.warmup, stream,
+                          measured["tuned"]["results"])
+        parse["gbps_uncompressed"] = round(uncomp / parse["avg_kernel_ms"] / 1e6, 2)
+        parse["algo_gbps"] = round(algo / parse["avg_kernel_ms"] / 1e6, 2)
+        parse["decode_over_parse"] = round(measured["tuned"]["avg_kernel_ms"] /
+                                           parse["avg_kernel_ms"], 3)
+        log(f"parse only: {parse}")
     graph = None
     if "tuned" in measured and not args.no_graph:
         graph = graph_leg(db, args.format, args.steps, stream, orig_dev, batch, valid,
                           measured["tuned"]["results"])
         log(f"graph: {graph['avg_kernel_ms']:.3f} ms/replay over {graph['replays']} replays, verified")
     cpu = None
+    cpu_config = None
     malformed_checked = False
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        cpu, cpu_status, cpu_written = cpu_baseline(batch, args.format, threads, valid)
+        threads, visible = cpu_threads(args.cpu_threads)
+        cpu, cpu_status, cpu_written = cpu_baseline(batch, args.format, threads, valid, visible)
         log(f"cpu baseline: {cpu}")
+        if args.format == "lz4":
+            cpu_config = cpu_config_token64k(threads, args.seed)
+            log(f"cpu configuration: {cpu_config}")
         # every chunk, malformed ones included: the GPU's status and
         # bytes_written equal the CPU reference's
         for form in forms:
@@ -524,6 +644,14 @@ def main():
                          "algo_gbps": round(measured["plain"]["algo_gbps"], 2)}
         line["speedup_vs_plain"] = round(measured["plain"]["avg_kernel_ms"] / head["avg_kernel_ms"], 3)
     line["avg_kernel_ms"] = round(head["avg_kernel_ms"], 4)
+    line["kernel_ms_percentiles"] = {
+        "p50": round(head["p50_kernel_ms"], 4), "p90": round(head["p90_kernel_ms"], 4),
+        "p99": round(head["p99_kernel_ms"], 4), "launches": args.steps,
+        "rule": "nearest rank over the event-timed launches"}
+    if parse is not None:
+        line["parse_only"] = parse
+    if cpu_config is not None:
+        line["cpu_config_token64k"] = cpu_config
     if graph is not None:
         line["graph"] = graph
     if tuning is not None:

@@ -16,12 +16,14 @@ static_assert(kFormatLz4 == CUDEC_MI355X_FORMAT_LZ4 && kFormatSnappy == CUDEC_MI
               "one format numbering");
 
 /* Launch-form selector. */
-enum Form : int { kFormPlain = 0, kFormTuned = 1, kFormTunedFixed = 2 };
+enum Form : int { kFormPlain = 0, kFormTuned = 1, kFormTunedFixed = 2, kFormParseOnly = 3 };
 
+/* parse_only: the parse-only build (statuses and sizes, no output bytes) */
 hipError_t launch_tuned(int format, const void* const* srcs,
                         const size_t* src_sizes, void* const* dsts,
                         const size_t* caps, size_t n,
-                        cudec_chunk_result* results, hipStream_t stream);
+                        cudec_chunk_result* results, hipStream_t stream,
+                        bool parse_only);
 /* launch_tuned with the fixed per-device scratch array (n <=
  * CUDEC_MI355X_NOALLOC_MAX_CHUNKS): no allocation. */
 hipError_t launch_tuned_fixed(int format, const void* const* srcs,
@@ -51,15 +53,18 @@ static cudec_status submit(int format, int form, const void* const* srcs,
     if (valid != CUDEC_OK) {
         return valid;
     }
-    if (form == kFormTunedFixed && n > CUDEC_MI355X_NOALLOC_MAX_CHUNKS) {
+    if ((form == kFormTunedFixed && n > CUDEC_MI355X_NOALLOC_MAX_CHUNKS) ||
+        (form == kFormParseOnly && n > UINT32_MAX)) {
         return CUDEC_ERR_INVALID_ARGUMENT;
     }
     (void)hipGetLastError();
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const hipError_t err =
-        form == kFormPlain   ? launch_plain(format, srcs, sizes, dsts, caps, n, results, s)
-        : form == kFormTuned ? launch_tuned(format, srcs, sizes, dsts, caps, n, results, s)
-                             : launch_tuned_fixed(format, srcs, sizes, dsts, caps, n, results, s);
+        form == kFormPlain ? launch_plain(format, srcs, sizes, dsts, caps, n, results, s)
+        : form == kFormTunedFixed
+            ? launch_tuned_fixed(format, srcs, sizes, dsts, caps, n, results, s)
+            : launch_tuned(format, srcs, sizes, dsts, caps, n, results, s,
+                           form == kFormParseOnly);
     return err == hipSuccess ? CUDEC_OK : CUDEC_ERR_CUDA;
 }
 
@@ -116,6 +121,17 @@ cudec_status cudec_mi355x_lz4_decompress_batch_noalloc(
     return submit(kFormatLz4, kFormTunedFixed, d_src_ptrs, d_src_sizes,
                   d_dst_ptrs, d_dst_capacities, chunk_count, d_results,
                   stream);
+}
+
+cudec_status cudec_mi355x_parse_batch(int format, const void* const* d_src_ptrs,
+                                      const size_t* d_src_sizes, void* const* d_dst_ptrs,
+                                      const size_t* d_dst_capacities, size_t chunk_count,
+                                      cudec_chunk_result* d_results, cudec_stream_t stream) {
+    if (format != kFormatLz4 && format != kFormatSnappy) {
+        return CUDEC_ERR_INVALID_ARGUMENT;
+    }
+    return submit(format, kFormParseOnly, d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                  d_dst_capacities, chunk_count, d_results, stream);
 }
 
 cudec_status cudec_snappy_decompress_batch_plain(

@@ -1759,8 +1759,9 @@ struct ChunkOut {
     uint64_t produced;
 };
 
-/* One wave per chunk: parse a window, then execute it, in turn. */
-template <int kFormat>
+/* One wave per chunk: parse a window, then execute it, in turn.
+ * kParseOnly: parse only (statuses and sizes, no output bytes). */
+template <int kFormat, bool kParseOnly>
 __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
                                                  uint64_t src_size, gu8* dst, uint64_t cap,
                                                  uint32_t lane) {
@@ -1780,7 +1781,9 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
             status = pr.status;
             break;
         }
-        if (pr.kind == kParseGroup) {
+        if (kParseOnly) {
+            dp = pdp;
+        } else if (pr.kind == kParseGroup) {
             if (pr.lead) {
                 execute_slow(w, src, dst, cap, pr.q, pr.lead_dp, dp, flushed, ring_lo, false,
                              lane);
@@ -1952,7 +1955,7 @@ constexpr uint32_t kNoFin = 0xFFFFFFFFu;
  * throughput-bound and gain from a fifth wave. kSubs: decode sub-queues,
  * kSubQueues for launches of many chunks per wave, else 1 (the single
  * counter, whose code keeps the fewest scalar registers live). */
-template <int kFormat, int kWaves, uint32_t kSubs>
+template <int kFormat, int kWaves, uint32_t kSubs, bool kParseOnly>
 __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu(kWaves)))
     piped_batch(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
                 const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
@@ -2048,6 +2051,14 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                 const uint32_t kind = uniform32(r.kind);
                 if (kind == kParseFail) {
                     status = (int)uniform32((uint32_t)r.status);
+                } else if (kParseOnly) {
+                    /* the parse ceiling: the records' sizes, no bytes */
+                    if (kind == kParseSlow || uniform32(r.lead) != 0) {
+                        dp = uniform64(r.dp_after);
+                    }
+                    if (kind == kParseGroup) {
+                        dp += uniform32(r.end_rel);
+                    }
                 } else {
                     if (kind == kParseSlow || uniform32(r.lead) != 0) {
                         Lz4Step s;
@@ -2101,7 +2112,7 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
         gu8* dst = global_bytes(dsts[chunk]);
         const uint64_t src_size = uniform64(src_sizes[chunk]);
         const uint64_t cap = uniform64(caps[chunk]);
-        const ChunkOut o = decode_chunk<kFormat>(w, src, src_size, dst, cap, lane);
+        const ChunkOut o = decode_chunk<kFormat, kParseOnly>(w, src, src_size, dst, cap, lane);
         write_result(results, chunk, (int)uniform32((uint32_t)o.status), uniform64(o.produced));
         idx = queue_pop<kSubs>(q, n, cursor, lane);
     }
@@ -2167,13 +2178,13 @@ hipError_t device_state(DeviceState** out) {
             return e;
         }
         int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 4, 1>,
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 4, 1, false>,
                                                          kPipeBlock, 0);
         if (e != hipSuccess) {
             return e;
         }
         d.pipe_blocks_per_cu = per > 0 ? per : 1;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 5, 1>,
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, piped_batch<kFormatLz4, 5, 1, false>,
                                                          kPipeBlock, 0);
         if (e != hipSuccess) {
             return e;
@@ -2292,11 +2303,12 @@ uint32_t host_cost_bin(uint64_t src_size, uint64_t cap) {
 namespace {
 
 /* Bins the batch into the queue held by `work` (a Queue, then order[n])
- * and launches the persistent decoder, all on `stream`. */
+ * and launches the persistent decoder, all on `stream`; parse_only
+ * launches its parse-only build. */
 hipError_t queue_and_decode(DeviceState* dev, int format, const void* const* srcs,
                             const size_t* src_sizes, void* const* dsts, const size_t* caps,
                             size_t n, cudec_chunk_result* results, void* work,
-                            hipStream_t stream) {
+                            hipStream_t stream, bool parse_only) {
     Plan plan;
     {
         std::lock_guard<std::mutex> hold(g_devices_lock);
@@ -2327,19 +2339,26 @@ hipError_t queue_and_decode(DeviceState* dev, int format, const void* const* src
     }
     const dim3 gd((unsigned)grid), bd(kPipeBlock);
     const bool many = (uint64_t)n >= kSubQueueMinPerWave * 2 * (uint64_t)grid;
-    const int shape = (format == kFormatSnappy ? 4 : 0) | (five_waves ? 2 : 0) | (many ? 1 : 0);
+    const int shape = (parse_only ? 8 : 0) | (format == kFormatSnappy ? 4 : 0) |
+                      (five_waves ? 2 : 0) | (many ? 1 : 0);
     switch (shape) {
-#define CUDEC_PIPED(kShape, kFmt, kW, kS)                                                   \
-    case kShape:                                                                            \
-        hipLaunchKernelGGL((piped_batch<kFmt, kW, kS>), gd, bd, 0, stream, srcs, src_sizes, \
-                           dsts, caps, n, results, q, order, plan);                         \
+#define CUDEC_PIPED(kShape, kFmt, kW, kS, kP)                                             \
+    case kShape:                                                                          \
+        hipLaunchKernelGGL((piped_batch<kFmt, kW, kS, kP>), gd, bd, 0, stream, srcs,      \
+                           src_sizes, dsts, caps, n, results, q, order, plan);            \
         break;
-        CUDEC_PIPED(0, kFormatLz4, 4, 1)
-        CUDEC_PIPED(1, kFormatLz4, 4, kSubQueues)
-        CUDEC_PIPED(2, kFormatLz4, 5, 1)
-        CUDEC_PIPED(3, kFormatLz4, 5, kSubQueues)
-        CUDEC_PIPED(6, kFormatSnappy, 5, 1)
-        CUDEC_PIPED(7, kFormatSnappy, 5, kSubQueues)
+        CUDEC_PIPED(0, kFormatLz4, 4, 1, false)
+        CUDEC_PIPED(1, kFormatLz4, 4, kSubQueues, false)
+        CUDEC_PIPED(2, kFormatLz4, 5, 1, false)
+        CUDEC_PIPED(3, kFormatLz4, 5, kSubQueues, false)
+        CUDEC_PIPED(6, kFormatSnappy, 5, 1, false)
+        CUDEC_PIPED(7, kFormatSnappy, 5, kSubQueues, false)
+        CUDEC_PIPED(8, kFormatLz4, 4, 1, true)
+        CUDEC_PIPED(9, kFormatLz4, 4, kSubQueues, true)
+        CUDEC_PIPED(10, kFormatLz4, 5, 1, true)
+        CUDEC_PIPED(11, kFormatLz4, 5, kSubQueues, true)
+        CUDEC_PIPED(14, kFormatSnappy, 5, 1, true)
+        CUDEC_PIPED(15, kFormatSnappy, 5, kSubQueues, true)
 #undef CUDEC_PIPED
         default: /* Snappy always takes the five-wave build */
             return hipErrorInvalidValue;
@@ -2351,11 +2370,12 @@ hipError_t queue_and_decode(DeviceState* dev, int format, const void* const* src
 
 hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_sizes,
                         void* const* dsts, const size_t* caps, size_t n,
-                        cudec_chunk_result* results, hipStream_t stream) {
+                        cudec_chunk_result* results, hipStream_t stream, bool parse_only) {
     if (n > UINT32_MAX) {
         /* the queue indexes chunks in 32 bits; larger batches take the
-         * plain form, which has no such limit */
-        return launch_plain(format, srcs, src_sizes, dsts, caps, n, results, stream);
+         * plain form, which has no such limit (and no parse-only build) */
+        return parse_only ? hipErrorInvalidValue
+                          : launch_plain(format, srcs, src_sizes, dsts, caps, n, results, stream);
     }
     DeviceState* dev = nullptr;
     hipError_t e = device_state(&dev);
@@ -2383,7 +2403,8 @@ hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_s
     if (e != hipSuccess) {
         return e;
     }
-    e = queue_and_decode(dev, format, srcs, src_sizes, dsts, caps, n, results, mem, stream);
+    e = queue_and_decode(dev, format, srcs, src_sizes, dsts, caps, n, results, mem, stream,
+                         parse_only);
     const hipError_t f = hipFreeAsync(mem, stream);
     return e != hipSuccess ? e : f;
 }
@@ -2403,7 +2424,7 @@ hipError_t launch_tuned_fixed(int format, const void* const* srcs, const size_t*
         return dev->fixed_status != hipSuccess ? dev->fixed_status : hipErrorInvalidSymbol;
     }
     return queue_and_decode(dev, format, srcs, src_sizes, dsts, caps, n, results, dev->fixed,
-                            stream);
+                            stream, false);
 }
 
 }  // namespace cudec_amd

@@ -20,6 +20,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* ------------------------------------------------------------------ PRNG */
 
@@ -457,8 +458,11 @@ static void load_codecs(void) {
         Dl_info info;
         if (g_codecs.snappy_compress &&
             dladdr((void*)g_codecs.snappy_compress, &info) && info.dli_fname) {
+            /* the file the soname link names carries the version */
+            char* real = realpath(info.dli_fname, NULL);
             snprintf(g_codecs.snappy_path, sizeof g_codecs.snappy_path, "%s",
-                     info.dli_fname);
+                     real ? real : info.dli_fname);
+            free(real);
         }
     }
 }
@@ -613,4 +617,191 @@ uint64_t wl_fnv1a64(uint64_t hash, const uint8_t* data, uint64_t n) {
         hash *= 1099511628211ull;
     }
     return hash;
+}
+
+/* ------------------------------------------------------- CPU baseline */
+
+/* Decoders the CPU baseline times. The twins are the reference's host
+ * ladders compiled from its source (or their C restatement), passed in as
+ * a function pointer; the libraries are the ones loaded above. */
+enum {
+    WL_DEC_LIBLZ4 = 0,      /* LZ4_decompress_safe */
+    WL_DEC_LIBSNAPPY = 1,   /* snappy_uncompress (capacity checked first) */
+    WL_DEC_TWIN_LZ4 = 2,    /* int32 fn(src, size, dst, cap, uint64* written) */
+    WL_DEC_TWIN_SNAPPY = 3, /* int32 fn(src, size, dst, cap, uint64* written, int32* rung) */
+};
+typedef int32_t (*twin_lz4_fn)(const uint8_t*, uint64_t, uint8_t*, uint64_t, uint64_t*);
+typedef int32_t (*twin_snappy_fn)(const uint8_t*, uint64_t, uint8_t*, uint64_t, uint64_t*,
+                                  int32_t*);
+typedef int (*snappy_length_fn)(const char*, size_t, size_t*);
+
+typedef struct {
+    int32_t kind;
+    void* fn;
+    const uint8_t* const* srcs;
+    const uint64_t* sizes;
+    uint8_t* const* dsts;
+    const uint64_t* caps;
+    uint64_t begin, end;
+    int32_t* status;   /* 0 OK, else nonzero */
+    uint64_t* written;
+    int32_t passes;
+    atomic_int* go;    /* passes released so far */
+    atomic_int* done;  /* partitions finished, over all passes */
+} time_part;
+
+static snappy_length_fn g_snappy_length;
+
+static void decode_part(time_part* t) {
+    for (uint64_t i = t->begin; i < t->end; i++) {
+        uint64_t w = 0;
+        int32_t st = 0;
+        switch (t->kind) {
+            case WL_DEC_LIBLZ4: {
+                const int r = g_codecs.lz4_decompress((const char*)t->srcs[i], (char*)t->dsts[i],
+                                                      (int)t->sizes[i], (int)t->caps[i]);
+                st = r < 0 ? 1 : 0;
+                w = r < 0 ? 0 : (uint64_t)r;
+                break;
+            }
+            case WL_DEC_LIBSNAPPY: {
+                size_t len = 0;
+                if (g_snappy_length((const char*)t->srcs[i], (size_t)t->sizes[i], &len) != 0 ||
+                    len > t->caps[i]) {
+                    st = 1;
+                    break;
+                }
+                len = (size_t)t->caps[i];
+                st = g_codecs.snappy_uncompress((const char*)t->srcs[i], (size_t)t->sizes[i],
+                                                (char*)t->dsts[i], &len) != 0;
+                w = st ? 0 : (uint64_t)len;
+                break;
+            }
+            case WL_DEC_TWIN_LZ4:
+                st = ((twin_lz4_fn)t->fn)(t->srcs[i], t->sizes[i], t->dsts[i], t->caps[i], &w);
+                break;
+            default: {
+                int32_t rung = 0;
+                st = ((twin_snappy_fn)t->fn)(t->srcs[i], t->sizes[i], t->dsts[i], t->caps[i], &w,
+                                             &rung);
+                break;
+            }
+        }
+        if (t->status) {
+            t->status[i] = st;
+        }
+        if (t->written) {
+            t->written[i] = st == 0 ? w : 0;
+        }
+    }
+}
+
+/* A worker waits for each pass to be released (spinning: the wait sits
+ * between passes, outside the decode), decodes its partition and counts
+ * itself done. */
+static void* time_worker(void* arg) {
+    time_part* t = (time_part*)arg;
+    for (int32_t p = 0; p < t->passes; p++) {
+        while (atomic_load_explicit(t->go, memory_order_acquire) <= p) {
+        }
+        decode_part(t);
+        atomic_fetch_add_explicit(t->done, 1, memory_order_release);
+    }
+    return NULL;
+}
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Times `passes` passes of one decoder over a batch of n chunks on `threads`
+ * threads (static contiguous partition; the threads are started before the
+ * first pass and wait between passes, so a pass times decode calls only,
+ * into pre-sized buffers). secs[p] receives pass p's wall time; status and
+ * written (either may be NULL) receive every chunk's outcome (0 = OK) of
+ * the last pass. A partition whose thread could not start is decoded by
+ * the calling thread. Returns the number of threads that ran, or -1 when
+ * the decoder's library is absent. */
+int32_t wl_time_decode(int32_t kind, void* fn, const uint8_t* const* srcs,
+                       const uint64_t* sizes, uint8_t* const* dsts, const uint64_t* caps,
+                       uint64_t n, int32_t threads, int32_t passes, double* secs,
+                       int32_t* status, uint64_t* written) {
+    pthread_once(&g_codecs_once, load_codecs);
+    if (kind == WL_DEC_LIBLZ4 && !g_codecs.lz4_decompress) {
+        return -1;
+    }
+    if (kind == WL_DEC_LIBSNAPPY) {
+        if (!g_codecs.snappy_uncompress || !g_codecs.snappy) {
+            return -1;
+        }
+        g_snappy_length =
+            (snappy_length_fn)dlsym(g_codecs.snappy, "snappy_uncompressed_length");
+        if (!g_snappy_length) {
+            return -1;
+        }
+    }
+    if ((kind == WL_DEC_TWIN_LZ4 || kind == WL_DEC_TWIN_SNAPPY) && !fn) {
+        return -1;
+    }
+    if (threads < 1) {
+        threads = 1;
+    }
+    if ((uint64_t)threads > n && n > 0) {
+        threads = (int32_t)n;
+    }
+    time_part* parts = (time_part*)calloc((size_t)threads, sizeof(time_part));
+    pthread_t* tid = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+    int* started = (int*)calloc((size_t)threads, sizeof(int));
+    if (!parts || !tid || !started) {
+        free(parts);
+        free(tid);
+        free(started);
+        return -1;
+    }
+    atomic_int go, done;
+    atomic_init(&go, 0);
+    atomic_init(&done, 0);
+    int32_t running = 0;
+    for (int32_t t = 0; t < threads; t++) {
+        time_part* p = &parts[t];
+        p->kind = kind;
+        p->fn = fn;
+        p->srcs = srcs;
+        p->sizes = sizes;
+        p->dsts = dsts;
+        p->caps = caps;
+        p->begin = n * (uint64_t)t / (uint64_t)threads;
+        p->end = n * (uint64_t)(t + 1) / (uint64_t)threads;
+        p->status = status;
+        p->written = written;
+        p->passes = passes;
+        p->go = &go;
+        p->done = &done;
+        /* partition 0 is the calling thread's */
+        started[t] = t != 0 && pthread_create(&tid[t], NULL, time_worker, p) == 0;
+        running += started[t];
+    }
+    for (int32_t p = 0; p < passes; p++) {
+        const double t0 = now_s();
+        atomic_store_explicit(&go, p + 1, memory_order_release);
+        for (int32_t t = 0; t < threads; t++) {
+            if (!started[t]) {
+                decode_part(&parts[t]);
+            }
+        }
+        while (atomic_load_explicit(&done, memory_order_acquire) < (p + 1) * running) {
+        }
+        secs[p] = now_s() - t0;
+    }
+    for (int32_t t = 0; t < threads; t++) {
+        if (started[t]) {
+            pthread_join(tid[t], NULL);
+        }
+    }
+    free(parts);
+    free(tid);
+    free(started);
+    return running + 1;
 }

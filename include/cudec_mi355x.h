@@ -128,6 +128,18 @@ cudec_status cudec_mi355x_lz4_decompress_batch_noalloc(const void* const* d_src_
                                                        cudec_chunk_result* d_results,
                                                        cudec_stream_t stream);
 
+/* The parse ceiling, for measurement (the reference keeps the same arm:
+ * lz4_decode_batch<ParseOnly>, timed beside the full decode by its GPU
+ * bench). The format's tuned batch launch (format: CUDEC_MI355X_FORMAT_*)
+ * with every window parsed and every status decided, and nothing copied:
+ * d_results receives the same statuses and bytes_written as the format's
+ * batch entry, and no destination byte is written. Arguments and
+ * validation as the batch entry; at most 2^32 - 1 chunks. */
+cudec_status cudec_mi355x_parse_batch(int format, const void* const* d_src_ptrs,
+                                      const size_t* d_src_sizes, void* const* d_dst_ptrs,
+                                      const size_t* d_dst_capacities, size_t chunk_count,
+                                      cudec_chunk_result* d_results, cudec_stream_t stream);
+
 /* No counterpart in the reference: a 64-bit digest of each chunk's bytes,
  * computed on the device, for checking decoded output in place.
  * Chunk i covers d_ptrs[i][0, len_i), where len_i is d_results[i]

@@ -339,3 +339,13 @@ def test_noalloc_rejects_without_a_device_call():
     assert fn(16, 16, 16, 16, 0, 16, None) == I
     assert fn(None, 16, 16, 16, 1, 16, None) == I
     assert fn(16, 16, 16, 16, 1, 24, None) == I
+
+
+def test_parse_batch_rejects_without_a_device_call():
+    fn = cudec_amd.lib().cudec_mi355x_parse_batch
+    I = cudec_amd.CUDEC_ERR_INVALID_ARGUMENT
+    assert fn(2, 16, 16, 16, 16, 1, 16, None) == I             # unknown format
+    assert fn(0, 16, 16, 16, 16, 0, 16, None) == I
+    assert fn(1, None, 16, 16, 16, 1, 16, None) == I
+    assert fn(0, 16, 16, 16, 16, 1, 24, None) == I
+    assert fn(0, 16, 16, 16, 16, 1 << 32, 16, None) == I        # the queue indexes in 32 bits

@@ -35,7 +35,7 @@ CASES = {
 }
 # the compressed digests were taken with these library builds
 PINNED_VERSIONS = {workload.CODEC_LZ4: "liblz4 1.9.3",
-                   workload.CODEC_SNAPPY: "libsnappy (/opt/conda/lib/libsnappy.so.1)"}
+                   workload.CODEC_SNAPPY: "libsnappy (/opt/conda/lib/libsnappy.so.1.1.8)"}
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
