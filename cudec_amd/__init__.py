@@ -50,7 +50,7 @@ ABI_SYMBOLS = {
         "cudec_mi355x_set_snappy_pipeline_threshold", "cudec_mi355x_snappy_pipeline_threshold",
         "cudec_mi355x_autotune_lz4", "cudec_mi355x_autotune_plan",
         "cudec_mi355x_lz4_decompress_batch_noalloc", "cudec_mi355x_parse_batch",
-        "cudec_mi355x_digest_batch",
+        "cudec_mi355x_xxh32_batch", "cudec_mi355x_xxh64_batch", "cudec_mi355x_digest_batch",
         "cudec_mi355x_serve_lz4",
     ],
 }
@@ -101,6 +101,10 @@ def lib() -> ctypes.CDLL:
         L.cudec_lz4f_decompress.restype = ctypes.c_int
         L.cudec_mi355x_parse_batch.argtypes = [ctypes.c_int] + _BATCH_ARGS
         L.cudec_mi355x_parse_batch.restype = ctypes.c_int
+        L.cudec_mi355x_xxh32_batch.argtypes = [_vp, _vp, _sz, ctypes.c_uint32, _vp, _vp]
+        L.cudec_mi355x_xxh32_batch.restype = ctypes.c_int
+        L.cudec_mi355x_xxh64_batch.argtypes = [_vp, _vp, _sz, ctypes.c_uint64, _vp, _vp]
+        L.cudec_mi355x_xxh64_batch.restype = ctypes.c_int
         L.cudec_mi355x_digest_batch.argtypes = [_vp, _vp, _vp, _sz, _vp, _vp]
         L.cudec_mi355x_digest_batch.restype = ctypes.c_int
         L.cudec_mi355x_serve_lz4.argtypes = [_vp, _vp, _vp, _sz, _vp, _sz, _sz, ctypes.c_int,

@@ -14,7 +14,16 @@
  * per run of blocks that are contiguous both on the device and in the
  * output -- every block but the last decodes to exactly the block maximum,
  * so a frame of compressed blocks comes back in one copy. The caller's
- * capacity is checked before any output byte is written. */
+ * capacity is checked before any output byte is written.
+ *
+ * Checksums: block checksums (XXH32 of each block's stored bytes) are
+ * computed on the device, every block in parallel, from the same upload
+ * (stored blocks are uploaded too when the frame carries them), and
+ * checked in block order before any decode result is used. The header
+ * checksum (a few bytes) and the content checksum are computed on the
+ * host: the content checksum is one XXH32 stream over the whole output,
+ * and its stripes form a serial chain, which one host core hashes at
+ * about 13 GB/s and one group of GPU lanes at about 0.5 GB/s. */
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -22,6 +31,7 @@
 #include <vector>
 
 #include "cudec.h"
+#include "cudec_mi355x.h"
 #include "hip_owner.h"
 
 namespace cudec_amd {
@@ -87,49 +97,66 @@ struct Block {
     size_t off;
     size_t len;
     bool stored;  /* the high bit: the block is stored uncompressed */
+    uint32_t sum; /* the block checksum the frame carries (when it does) */
 };
 
-/* Decodes the compressed blocks in one batch, then assembles the output. */
+/* Checks the block checksums on the device (when `sums`), decodes the
+ * compressed blocks in one batch, then assembles the output. */
 cudec_status decode_blocks(const unsigned char* f, const std::vector<Block>& blocks,
-                           size_t block_max, unsigned char* out, size_t cap,
+                           size_t block_max, bool sums, unsigned char* out, size_t cap,
                            size_t* total_out) {
-    std::vector<size_t> ci;
+    /* the blocks that go to the device: the compressed ones, and every one
+     * when their checksums are checked there */
+    std::vector<size_t> up, ci;
     size_t total_src = 0;
     for (size_t i = 0; i < blocks.size(); i++) {
-        if (!blocks[i].stored) {
-            ci.push_back(i);
+        if (sums || !blocks[i].stored) {
+            up.push_back(i);
             total_src += blocks[i].len;
         }
+        if (!blocks[i].stored) {
+            ci.push_back(i);
+        }
     }
-    const size_t n = ci.size();
+    const size_t n = ci.size(), nu = up.size();
     std::vector<size_t> decoded(blocks.size(), 0);
     Stream stream;
-    DeviceRegion d_src, d_dst, d_meta, d_res;
-    if (n != 0) {
-        if (n > SIZE_MAX / block_max || n > SIZE_MAX / (4 * sizeof(void*))) {
+    DeviceRegion d_src, d_dst, d_meta, d_res, d_sums;
+    if (nu != 0) {
+        if (n > SIZE_MAX / block_max || nu > SIZE_MAX / (6 * sizeof(void*))) {
             return CUDEC_ERR_CORRUPT_INPUT;
         }
+        /* tables: decode sources, sizes, destinations, capacities (n
+         * each), then checksum sources and sizes (nu each) */
+        const size_t stride = n * sizeof(void*), ustride = nu * sizeof(void*);
         if (make_stream(stream) != hipSuccess || d_src.reserve(total_src) != hipSuccess ||
             d_dst.reserve(n * block_max) != hipSuccess ||
-            d_meta.reserve(4 * n * sizeof(void*)) != hipSuccess ||
-            d_res.reserve(n * sizeof(cudec_chunk_result)) != hipSuccess) {
+            d_meta.reserve(4 * stride + 2 * ustride) != hipSuccess ||
+            d_res.reserve(n * sizeof(cudec_chunk_result)) != hipSuccess ||
+            d_sums.reserve(nu * sizeof(uint32_t)) != hipSuccess) {
             return CUDEC_ERR_CUDA;
         }
         std::vector<unsigned char> stage(total_src);
-        const size_t stride = n * sizeof(void*);
-        std::vector<unsigned char> meta(4 * stride);
+        std::vector<unsigned char> meta(4 * stride + 2 * ustride);
         unsigned char* ds = static_cast<unsigned char*>(d_src.data());
         unsigned char* dd = static_cast<unsigned char*>(d_dst.data());
-        size_t so = 0;
-        for (size_t k = 0; k < n; k++) {
-            const Block& b = blocks[ci[k]];
+        size_t so = 0, k = 0;
+        for (size_t u = 0; u < nu; u++) {
+            const Block& b = blocks[up[u]];
             std::memcpy(stage.data() + so, f + b.off, b.len);
             const void* sp = ds + so;
-            void* dp = dd + k * block_max;
-            std::memcpy(meta.data() + k * sizeof(void*), &sp, sizeof(void*));
-            std::memcpy(meta.data() + stride + k * sizeof(size_t), &b.len, sizeof(size_t));
-            std::memcpy(meta.data() + 2 * stride + k * sizeof(void*), &dp, sizeof(void*));
-            std::memcpy(meta.data() + 3 * stride + k * sizeof(size_t), &block_max, sizeof(size_t));
+            std::memcpy(meta.data() + 4 * stride + u * sizeof(void*), &sp, sizeof(void*));
+            std::memcpy(meta.data() + 4 * stride + ustride + u * sizeof(size_t), &b.len,
+                        sizeof(size_t));
+            if (!b.stored) {
+                void* dp = dd + k * block_max;
+                std::memcpy(meta.data() + k * sizeof(void*), &sp, sizeof(void*));
+                std::memcpy(meta.data() + stride + k * sizeof(size_t), &b.len, sizeof(size_t));
+                std::memcpy(meta.data() + 2 * stride + k * sizeof(void*), &dp, sizeof(void*));
+                std::memcpy(meta.data() + 3 * stride + k * sizeof(size_t), &block_max,
+                            sizeof(size_t));
+                k++;
+            }
             so += b.len;
         }
         hipStream_t st = stream.get();
@@ -140,15 +167,31 @@ cudec_status decode_blocks(const unsigned char* f, const std::vector<Block>& blo
             return CUDEC_ERR_CUDA;
         }
         unsigned char* dm = static_cast<unsigned char*>(d_meta.data());
-        const cudec_status launched = cudec_lz4_decompress_batch(
-            reinterpret_cast<const void* const*>(dm), reinterpret_cast<const size_t*>(dm + stride),
-            reinterpret_cast<void* const*>(dm + 2 * stride),
-            reinterpret_cast<const size_t*>(dm + 3 * stride), n,
-            static_cast<cudec_chunk_result*>(d_res.data()), reinterpret_cast<cudec_stream_t>(st));
+        const cudec_stream_t cs = reinterpret_cast<cudec_stream_t>(st);
+        std::vector<uint32_t> got(nu);
+        cudec_status launched = CUDEC_OK;
+        bool copied = true;
+        if (sums) {
+            launched = cudec_mi355x_xxh32_batch(
+                reinterpret_cast<const void* const*>(dm + 4 * stride),
+                reinterpret_cast<const size_t*>(dm + 4 * stride + ustride), nu, 0,
+                static_cast<uint32_t*>(d_sums.data()), cs);
+            copied = launched == CUDEC_OK &&
+                     hipMemcpyAsync(got.data(), d_sums.data(), nu * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, st) == hipSuccess;
+        }
         std::vector<cudec_chunk_result> res(n);
-        const bool copied = launched == CUDEC_OK &&
-                            hipMemcpyAsync(res.data(), d_res.data(), n * sizeof(cudec_chunk_result),
-                                           hipMemcpyDeviceToHost, st) == hipSuccess;
+        if (launched == CUDEC_OK && n != 0) {
+            launched = cudec_lz4_decompress_batch(
+                reinterpret_cast<const void* const*>(dm),
+                reinterpret_cast<const size_t*>(dm + stride),
+                reinterpret_cast<void* const*>(dm + 2 * stride),
+                reinterpret_cast<const size_t*>(dm + 3 * stride), n,
+                static_cast<cudec_chunk_result*>(d_res.data()), cs);
+            copied = copied && launched == CUDEC_OK &&
+                     hipMemcpyAsync(res.data(), d_res.data(), n * sizeof(cudec_chunk_result),
+                                    hipMemcpyDeviceToHost, st) == hipSuccess;
+        }
         /* the staging vectors must outlive the asynchronous uploads */
         if (hipStreamSynchronize(st) != hipSuccess) {
             return CUDEC_ERR_CUDA;
@@ -158,6 +201,13 @@ cudec_status decode_blocks(const unsigned char* f, const std::vector<Block>& blo
         }
         if (!copied) {
             return CUDEC_ERR_CUDA;
+        }
+        if (sums) {
+            for (size_t u = 0; u < nu; u++) {
+                if (got[u] != blocks[up[u]].sum) {
+                    return CUDEC_ERR_CORRUPT_INPUT;
+                }
+            }
         }
         for (size_t k = 0; k < n; k++) {
             /* any block the decoder refuses (including one longer than the
@@ -268,19 +318,22 @@ cudec_status decode_frame(const unsigned char* f, size_t size, unsigned char* ou
         if (len == 0 || len > block_max || size - pos < len) {
             return CUDEC_ERR_CORRUPT_INPUT;
         }
+        uint32_t sum = 0;
         if (block_sum) {
-            if (size - pos - len < 4 || xxh32(f + pos, len) != le32(f + pos + len)) {
+            /* the checksum itself is checked on the device, with the others */
+            if (size - pos - len < 4) {
                 return CUDEC_ERR_CORRUPT_INPUT;
             }
+            sum = le32(f + pos + len);
         }
-        blocks.push_back(Block{pos, len, ((bs >> 31) & 1) != 0});
+        blocks.push_back(Block{pos, len, ((bs >> 31) & 1) != 0, sum});
         pos += len + (block_sum ? 4 : 0);
     }
     if (!ended) {
         return CUDEC_ERR_CORRUPT_INPUT;
     }
     size_t total = 0;
-    const cudec_status st = decode_blocks(f, blocks, block_max, out, cap, &total);
+    const cudec_status st = decode_blocks(f, blocks, block_max, block_sum, out, cap, &total);
     if (st != CUDEC_OK) {
         return st;
     }

@@ -140,6 +140,21 @@ cudec_status cudec_mi355x_parse_batch(int format, const void* const* d_src_ptrs,
                                       const size_t* d_dst_capacities, size_t chunk_count,
                                       cudec_chunk_result* d_results, cudec_stream_t stream);
 
+/* Device checksums (the reference computes them on the host,
+ * src/xxhash32.h and src/xxhash64.h): XXH32 and XXH64 with the given seed,
+ * as the xxHash specification defines them, of count segments: segment i
+ * is d_sizes[i] bytes at d_ptrs[i], and its hash lands in d_hashes[i].
+ * The LZ4 frame format checksums with XXH32 (seed 0), the Zstd frame
+ * format's content checksum is the low 32 bits of XXH64 (seed 0). Arrays
+ * are device memory; asynchronous on stream, like the batch entries.
+ * NULL arrays or count 0 are CUDEC_ERR_INVALID_ARGUMENT. */
+cudec_status cudec_mi355x_xxh32_batch(const void* const* d_ptrs, const size_t* d_sizes,
+                                      size_t count, uint32_t seed, uint32_t* d_hashes,
+                                      cudec_stream_t stream);
+cudec_status cudec_mi355x_xxh64_batch(const void* const* d_ptrs, const size_t* d_sizes,
+                                      size_t count, uint64_t seed, uint64_t* d_hashes,
+                                      cudec_stream_t stream);
+
 /* No counterpart in the reference: a 64-bit digest of each chunk's bytes,
  * computed on the device, for checking decoded output in place.
  * Chunk i covers d_ptrs[i][0, len_i), where len_i is d_results[i]

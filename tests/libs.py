@@ -313,3 +313,71 @@ def xxh32(data: bytes, seed: int = 0) -> int:
     h = (h * _P3) & _M32
     h ^= h >> 16
     return h
+
+
+_Q1, _Q2, _Q3, _Q4, _Q5 = (0x9E3779B185EBCA87, 0xC2B2AE3D27D4EB4F, 0x165667B19E3779F9,
+                           0x85EBCA77C2B2AE63, 0x27D4EB2F165667C5)
+_M64 = (1 << 64) - 1
+
+
+def _rotl64(x: int, r: int) -> int:
+    return ((x << r) | (x >> (64 - r))) & _M64
+
+
+def _round64(acc: int, lane: int) -> int:
+    return (_rotl64((acc + lane * _Q2) & _M64, 31) * _Q1) & _M64
+
+
+def xxh64(data: bytes, seed: int = 0) -> int:
+    """XXH64 from the published xxHash specification (four 64-bit lanes over
+    32-byte stripes, the lane merge, 8-, 4- and 1-byte tails, the
+    avalanche). Pinned against the specification's empty-input value and,
+    where the system's libxxhash is present, against it."""
+    n = len(data)
+    i = 0
+    if n >= 32:
+        v = [(seed + _Q1 + _Q2) & _M64, (seed + _Q2) & _M64, seed & _M64, (seed - _Q1) & _M64]
+        while i + 32 <= n:
+            for k in range(4):
+                v[k] = _round64(v[k], int.from_bytes(data[i + 8 * k:i + 8 * k + 8], "little"))
+            i += 32
+        h = (_rotl64(v[0], 1) + _rotl64(v[1], 7) + _rotl64(v[2], 12) + _rotl64(v[3], 18)) & _M64
+        for k in range(4):
+            h = ((h ^ _round64(0, v[k])) * _Q1 + _Q4) & _M64
+    else:
+        h = (seed + _Q5) & _M64
+    h = (h + n) & _M64
+    while i + 8 <= n:
+        h ^= _round64(0, int.from_bytes(data[i:i + 8], "little"))
+        h = (_rotl64(h, 27) * _Q1 + _Q4) & _M64
+        i += 8
+    if i + 4 <= n:
+        h ^= (int.from_bytes(data[i:i + 4], "little") * _Q1) & _M64
+        h = (_rotl64(h, 23) * _Q2 + _Q3) & _M64
+        i += 4
+    while i < n:
+        h ^= (data[i] * _Q5) & _M64
+        h = (_rotl64(h, 11) * _Q1) & _M64
+        i += 1
+    h ^= h >> 33
+    h = (h * _Q2) & _M64
+    h ^= h >> 29
+    h = (h * _Q3) & _M64
+    h ^= h >> 32
+    return h
+
+
+@lru_cache(None)
+def xxhash_lib():
+    """The system's libxxhash (XXH32, XXH64), or None where it is absent."""
+    for name in ("libxxhash.so.0", "/usr/lib/x86_64-linux-gnu/libxxhash.so.0"):
+        try:
+            lib = ctypes.CDLL(name)
+        except OSError:
+            continue
+        lib.XXH32.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32]
+        lib.XXH32.restype = ctypes.c_uint32
+        lib.XXH64.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64]
+        lib.XXH64.restype = ctypes.c_uint64
+        return lib
+    return None

@@ -197,12 +197,14 @@ def test_frame_host_verdicts():
 
 
 @needs_lz4
-@pytest.mark.parametrize("flags", [(False, False, False), (True, True, True), (True, False, True),
-                                   (False, True, False)])
+@pytest.mark.parametrize("flags", [(False, False, False), (True, False, True), (True, False, False),
+                                   (False, False, True)])
 def test_stored_block_frames_decode_on_host(flags):
-    """Incompressible data makes liblz4 store every block verbatim, so the
-    frame decodes with no device work at all: this pins XXH32 (header, block
-    and content checksums) and the block walk against liblz4's frames."""
+    """Incompressible data makes liblz4 store every block verbatim, so a
+    frame without block checksums decodes with no device work at all: this
+    pins XXH32 (header and content checksums) and the block walk against
+    liblz4's frames. Block checksums are checked on the device
+    (tests/test_gpu_frame.py)."""
     from tests import corpus
     cs, bs, sz = flags
     data = corpus.random_bytes(300000, 11)
@@ -349,3 +351,29 @@ def test_parse_batch_rejects_without_a_device_call():
     assert fn(1, None, 16, 16, 16, 1, 16, None) == I
     assert fn(0, 16, 16, 16, 16, 1, 24, None) == I
     assert fn(0, 16, 16, 16, 16, 1 << 32, 16, None) == I        # the queue indexes in 32 bits
+
+
+def test_test_side_xxh64_is_pinned():
+    """The specification's empty-input value, and the system's libxxhash
+    (an implementation independent of this build) on lengths that reach
+    every tail path, at two seeds; the same for the test-side XXH32."""
+    assert libs.xxh64(b"") == 0xEF46DB3751D8E999
+    x = libs.xxhash_lib()
+    if x is None:
+        pytest.skip("libxxhash absent")
+    rng = np.random.default_rng(5)
+    for n in list(range(0, 80)) + [255, 256, 1000, 4099]:
+        d = rng.integers(0, 256, n, dtype=np.uint8).tobytes()
+        for seed in (0, 0x9E3779B1):
+            assert libs.xxh64(d, seed) == x.XXH64(d, n, seed), (n, seed)
+            assert libs.xxh32(d, seed) == x.XXH32(d, n, seed), (n, seed)
+
+
+def test_checksum_entries_reject_without_a_device_call():
+    L = cudec_amd.lib()
+    I = cudec_amd.CUDEC_ERR_INVALID_ARGUMENT
+    for fn in (L.cudec_mi355x_xxh32_batch, L.cudec_mi355x_xxh64_batch):
+        assert fn(None, 16, 1, 0, 16, None) == I
+        assert fn(16, None, 1, 0, 16, None) == I
+        assert fn(16, 16, 0, 0, 16, None) == I
+        assert fn(16, 16, 1, 0, None, None) == I

@@ -206,3 +206,39 @@ def test_kernel_reject_status_passes_through():
         bytes(frame[off:off + blen - 1]) + b"\0\0\0\0"
     assert libs.lz4_frame_decode(cut, 1 << 20)[0] is False
     expect_reject(cut, CORRUPT)
+
+
+@pytest.mark.parametrize("content_sum", [False, True])
+def test_block_checksums_checked_on_the_device(content_sum):
+    """Block checksums are computed on the device for every block, stored
+    and compressed alike. Stored-block frames (incompressible data) and a
+    many-block frame decode when every checksum holds; one flipped data
+    byte or checksum byte anywhere makes the frame CORRUPT, and a bad
+    checksum wins over a capacity that is too small, as the host parse
+    order has it (checksums before any decode result)."""
+    from tests import corpus
+    stored = corpus.random_bytes(300000, 11)
+    mixed = make_data(40 * 65536 + 123, 0x4C, 2)
+    for data, bsid in ((stored, 4), (mixed, 4), (mixed, 5)):
+        frame = libs.lz4_frame(data, block_size_id=bsid, content_checksum=content_sum,
+                               block_checksum=True)
+        st, out = decode(frame, len(data))
+        assert st == OK and out == data
+        # blocks: walk the frame to find each block's data and checksum bytes
+        pos = 7
+        spans = []
+        while True:
+            bs = int.from_bytes(frame[pos:pos + 4], "little")
+            pos += 4
+            if bs == 0:
+                break
+            n = bs & 0x7FFFFFFF
+            spans.append((pos, n))
+            pos += n + 4
+        for k in sorted({0, len(spans) // 2, len(spans) - 1}):
+            off, n = spans[k]
+            for at in (off, off + n // 2, off + n + 1):        # data bytes, checksum byte
+                bad = bytearray(frame)
+                bad[at] ^= 0x10
+                expect_reject(bytes(bad), CORRUPT)
+                expect_reject(bytes(bad), CORRUPT, cap=len(data) // 3)
