@@ -73,6 +73,9 @@ def parse_args():
     p.add_argument("--serve-pool", type=int, default=2048,
                    help="distinct chunks the serving requests draw from")
     p.add_argument("--serve-budget-gib", type=float, default=64.0)
+    p.add_argument("--serve-inflight", default="2,1,4",
+                   help="launches in flight (streams) of the serving runs, comma-separated; "
+                        "the first is the headline configuration")
     return p.parse_args()
 
 
@@ -440,6 +443,45 @@ def committed_traffic():
     return int(val.group(1))
 
 
+def chunk_latency_leg(seed):
+    """Idle latency of one chunk per size bin: a batch of one token-process
+    chunk, decoded by the tuned launch on two pipelined waves and on one
+    wave, median of 10 event-timed launches after 2 warmups. A request's
+    latency cannot fall below its largest chunk's."""
+    import numpy as np
+    import torch
+    import cudec_amd
+    from cudec_amd import workload
+    from cudec_amd.device import DeviceBatch
+    st, old = cudec_amd.pipeline_threshold("lz4")
+    out = []
+    try:
+        for size in (4 << 10, 16 << 10, 64 << 10, 256 << 10, 1 << 20, 4 << 20):
+            b = workload.build(workload.CODEC_LZ4, np.array([size], np.uint64),
+                               np.zeros(1, np.int32), np.array([seed + size], np.uint64))
+            db = DeviceBatch(b.comp, b.comp_off, b.comp_size, b.sizes)
+            row = {"bytes": size}
+            for name, t in (("two_wave_ms", 0), ("one_wave_ms", (1 << 64) - 1)):
+                cudec_amd.set_pipeline_threshold("lz4", t)
+                ms = []
+                for r in range(12):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    assert db.run("lz4", "tuned") == 0
+                    e1.record()
+                    torch.cuda.synchronize()
+                    if r >= 2:
+                        ms.append(e0.elapsed_time(e1))
+                row[name] = round(float(np.median(ms)), 4)
+            res = db.results()
+            assert res["status"][0] == 0 and res["bytes_written"][0] == size
+            out.append(row)
+    finally:
+        cudec_amd.set_pipeline_threshold("lz4", old)
+    return out
+
+
 def serving_leg(args):
     """Poisson request streams through the native serving loop. Per rate:
     one run with device digests, checked chunk by chunk against the
@@ -459,39 +501,46 @@ def serving_leg(args):
     log(f"serving pool: {pool.n} chunks, {pool.total_uncompressed / 2**20:.0f} MiB out, "
         f"built in {time.perf_counter() - t0:.1f}s")
     budget = int(args.serve_budget_gib * 2**30)
+    inflights = [int(x) for x in args.serve_inflight.split(",") if x]
     out = []
-    for k, rate in enumerate(rates):
-        nreq = max(16, int(rate * args.serve_seconds))
-        req, ids = serve.make_trace(rate, nreq, pool.n, seed=args.seed + k)
-        check = serve.serve(d_comp.data_ptr(), pool, req, ids, budget=budget, inflight=4,
-                            digests=True)
-        ok = check.status == 0 and bool(np.all(check.results["status"] == 0)) and \
-            bool(np.array_equal(check.results["bytes_written"], pool.sizes[ids])) and \
-            bool(np.array_equal(check.digests, want[ids]))
-        if not ok:
-            raise SystemExit(f"[bench] serving at {rate:.0f}/s: verification FAILED")
-        run = serve.serve(d_comp.data_ptr(), pool, req, ids, budget=budget, inflight=4,
-                          digests=False)
-        if run.status != 0 or not np.array_equal(run.results, check.results):
-            raise SystemExit(f"[bench] serving at {rate:.0f}/s: timed run differs")
-        st = run.stats
-        algo = st.output_bytes + st.compressed_bytes
-        row = {"rate_per_s": rate, "requests": nreq, "chunks": int(len(ids)),
-               "p50_ms": round(1e3 * serve.percentile(run.latency_s, 50), 3),
-               "p99_ms": round(1e3 * serve.percentile(run.latency_s, 99), 3),
-               "gbps_uncompressed": round(st.output_bytes / st.elapsed_s / 1e9, 2),
-               "gbps_algorithmic": round(algo / st.elapsed_s / 1e9, 2),
-               "offered_gbps_uncompressed": round(st.output_bytes / max(
-                   float(req["arrival_s"][-1]), 1e-9) / 1e9, 2),
-               "launches": int(st.launches), "max_batch_requests": int(st.max_batch_requests),
-               "max_inflight": int(st.max_inflight),
-               "peak_pool_mib": round(st.peak_pool_bytes / 2**20, 1)}
-        log(f"serving {rate:.0f}/s: {row}")
-        out.append(row)
-    return {"budget_bytes": budget, "inflight": 4, "verified": True,
+    sweep = []
+    for inflight in inflights:
+        for k, rate in enumerate(rates):
+            nreq = max(16, int(rate * args.serve_seconds))
+            req, ids = serve.make_trace(rate, nreq, pool.n, seed=args.seed + k)
+            check = serve.serve(d_comp.data_ptr(), pool, req, ids, budget=budget,
+                                inflight=inflight, digests=True)
+            ok = check.status == 0 and bool(np.all(check.results["status"] == 0)) and \
+                bool(np.array_equal(check.results["bytes_written"], pool.sizes[ids])) and \
+                bool(np.array_equal(check.digests, want[ids]))
+            if not ok:
+                raise SystemExit(f"[bench] serving at {rate:.0f}/s: verification FAILED")
+            run = serve.serve(d_comp.data_ptr(), pool, req, ids, budget=budget,
+                              inflight=inflight, digests=False)
+            if run.status != 0 or not np.array_equal(run.results, check.results):
+                raise SystemExit(f"[bench] serving at {rate:.0f}/s: timed run differs")
+            st = run.stats
+            algo = st.output_bytes + st.compressed_bytes
+            row = {"rate_per_s": rate, "requests": nreq, "chunks": int(len(ids)),
+                   "p50_ms": round(1e3 * serve.percentile(run.latency_s, 50), 3),
+                   "p99_ms": round(1e3 * serve.percentile(run.latency_s, 99), 3),
+                   "gbps_uncompressed": round(st.output_bytes / st.elapsed_s / 1e9, 2),
+                   "gbps_algorithmic": round(algo / st.elapsed_s / 1e9, 2),
+                   "offered_gbps_uncompressed": round(st.output_bytes / max(
+                       float(req["arrival_s"][-1]), 1e-9) / 1e9, 2),
+                   "launches": int(st.launches),
+                   "max_batch_requests": int(st.max_batch_requests),
+                   "max_inflight": int(st.max_inflight),
+                   "peak_pool_mib": round(st.peak_pool_bytes / 2**20, 1)}
+            log(f"serving {rate:.0f}/s, {inflight} in flight: {row}")
+            if inflight == inflights[0]:
+                out.append(row)
+            sweep.append({"inflight": inflight, "rate_per_s": rate, "p50_ms": row["p50_ms"],
+                          "p99_ms": row["p99_ms"], "gbps_uncompressed": row["gbps_uncompressed"]})
+    return {"budget_bytes": budget, "inflight": inflights[0], "verified": True,
             "workload": f"{pool.n} distinct token chunks, log-uniform [4 KiB, 1 MiB); "
                         "requests of 1-256 chunks (log-uniform), Poisson arrivals",
-            "rates": out}
+            "rates": out, "inflight_sweep": sweep}
 
 
 def main():
@@ -607,6 +656,10 @@ def main():
                 raise SystemExit(f"[bench] {form}: results differ from the CPU reference")
         malformed_checked = True
     serving = serving_leg(args) if args.serve_rates else None
+    latency = None
+    if args.serve_rates and args.format == "lz4":
+        latency = chunk_latency_leg(args.seed)
+        log(f"single-chunk latency at idle: {latency}")
     traffic = None
     if args.traffic_file and os.path.exists(args.traffic_file):
         # HBM bytes per launch of the tuned kernel from rocprofv3 FETCH_SIZE and
@@ -662,6 +715,8 @@ def main():
                              "status_counts": {str(k): int(v) for k, v in zip(
                                  *np.unique(r["status"][malformed], return_counts=True))},
                              "checked_against_cpu_reference": malformed_checked}
+    if latency is not None:
+        line["single_chunk_latency_idle"] = latency
     if serving is not None:
         line["serving"] = serving
         line["p99_request_latency_ms"] = serving["rates"][-1]["p99_ms"]
