@@ -1,9 +1,9 @@
-"""cudec_amd -- MI355X (gfx950) batch LZ4 / Snappy block decoding behind the
-cudec C ABI.
+"""cudec_amd -- MI355X (gfx950) batch LZ4 / Snappy block and Zstandard frame
+decoding behind the cudec C ABI.
 
 The product is the native library cudec_amd/libcudec.so (HIP kernels and
-the C entry points declared in include/cudec.h, include/cudec_snappy.h and
-include/cudec_mi355x.h). This module binds those entry points with ctypes
+the C entry points declared in include/cudec.h, include/cudec_snappy.h,
+include/cudec_zstd.h and include/cudec_mi355x.h). This module binds those entry points with ctypes
 under their C names, so Python callers drive exactly the ABI a C caller
 does. Importing it loads the library and fails loudly if it is absent:
 there is no fallback decode path.
@@ -42,6 +42,7 @@ ABI_SYMBOLS = {
         "cudec_stream_ctx_destroy", "cudec_lz4_decompress_stream",
     ],
     "cudec_snappy.h": ["cudec_snappy_decompress_batch"],
+    "cudec_zstd.h": ["cudec_zstd_decompress_batch"],
     "cudec_mi355x.h": [
         "cudec_lz4_decompress_batch_plain", "cudec_snappy_decompress_batch_plain",
         "cudec_mi355x_max_batch_chunks", "cudec_mi355x_cost_bin",
@@ -82,7 +83,7 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(LIB_PATH)
         L.cudec_version.restype = ctypes.c_int
         for name in ("cudec_lz4_decompress_batch", "cudec_snappy_decompress_batch",
-                     "cudec_lz4_decompress_batch_plain",
+                     "cudec_zstd_decompress_batch", "cudec_lz4_decompress_batch_plain",
                      "cudec_snappy_decompress_batch_plain",
                      "cudec_mi355x_lz4_decompress_batch_noalloc"):
             fn = getattr(L, name)
@@ -157,6 +158,13 @@ def cudec_snappy_decompress_batch(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_cap
                   d_dst_capacities, chunk_count, d_results, stream)
 
 
+def cudec_zstd_decompress_batch(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
+                                chunk_count, d_results, stream=None) -> int:
+    """One Zstandard frame per chunk (include/cudec_zstd.h)."""
+    return _batch("cudec_zstd_decompress_batch", d_src_ptrs, d_src_sizes, d_dst_ptrs,
+                  d_dst_capacities, chunk_count, d_results, stream)
+
+
 def cudec_lz4_decompress_batch_plain(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
                                      chunk_count, d_results, stream=None) -> int:
     return _batch("cudec_lz4_decompress_batch_plain", d_src_ptrs, d_src_sizes, d_dst_ptrs,
@@ -179,6 +187,7 @@ ENTRY = {
     ("lz4", "plain"): cudec_lz4_decompress_batch_plain,
     ("snappy", "tuned"): cudec_snappy_decompress_batch,
     ("snappy", "plain"): cudec_snappy_decompress_batch_plain,
+    ("zstd", "tuned"): cudec_zstd_decompress_batch,
 }
 
 

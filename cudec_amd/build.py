@@ -37,7 +37,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}",
               f"-I{CSRC}", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
-HIP_SOURCES = ["abi.hip", "plain.hip", "tuned.hip", "digest.hip", "checksum.hip"]
+HIP_SOURCES = ["abi.hip", "plain.hip", "tuned.hip", "digest.hip", "checksum.hip", "zstd.hip"]
 HOST_SOURCES = ["stream.cpp", "frame.cpp", "serve.cpp", "tune.cpp"]
 
 

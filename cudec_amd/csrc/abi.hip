@@ -8,6 +8,7 @@
 
 #include "cudec_mi355x.h"
 #include "cudec_snappy.h"
+#include "cudec_zstd.h"
 
 namespace cudec_amd {
 
@@ -30,6 +31,11 @@ hipError_t launch_tuned_fixed(int format, const void* const* srcs,
                               const size_t* src_sizes, void* const* dsts,
                               const size_t* caps, size_t n,
                               cudec_chunk_result* results, hipStream_t stream);
+
+/* One wave per Zstandard frame (zstd.hip). */
+hipError_t launch_zstd(const void* const* srcs, const size_t* sizes, void* const* dsts,
+                       const size_t* caps, size_t n, cudec_chunk_result* results,
+                       hipStream_t stream);
 
 /* src/lz4_decode.cuh:187-201 with the wave64 bound. */
 static cudec_status validate(const void* const* srcs, const size_t* sizes,
@@ -100,6 +106,25 @@ cudec_status cudec_snappy_decompress_batch(const void* const* d_src_ptrs,
     return submit(kFormatSnappy, kFormTuned, d_src_ptrs, d_src_sizes,
                   d_dst_ptrs, d_dst_capacities, chunk_count, d_results,
                   stream);
+}
+
+cudec_status cudec_zstd_decompress_batch(const void* const* d_src_ptrs,
+                                         const size_t* d_src_sizes,
+                                         void* const* d_dst_ptrs,
+                                         const size_t* d_dst_capacities,
+                                         size_t chunk_count,
+                                         cudec_chunk_result* d_results,
+                                         cudec_stream_t stream) {
+    const cudec_status valid = validate(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
+                                        chunk_count, d_results);
+    if (valid != CUDEC_OK) {
+        return valid;
+    }
+    (void)hipGetLastError();
+    const hipError_t err = launch_zstd(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
+                                       chunk_count, d_results,
+                                       reinterpret_cast<hipStream_t>(stream));
+    return err == hipSuccess ? CUDEC_OK : CUDEC_ERR_CUDA;
 }
 
 cudec_status cudec_lz4_decompress_batch_plain(const void* const* d_src_ptrs,

@@ -1,0 +1,199 @@
+/* The Zstandard batch decoder for gfx950: one wave per frame, frames taken
+ * from a device counter by a persistent grid.
+ *
+ * A frame's decode (zstd_decode.h) keeps its Huffman and FSE tables in the
+ * wave's LDS (FrameTables, about 10 KiB), so a table built for one block is
+ * there for the next block's Treeless or Repeat mode. Huffman-coded
+ * literals are decoded four streams at a time, one lane a stream, into the
+ * wave's 128 KiB literals slot in global scratch; the sequences are decoded
+ * in order on every lane (the FSE states are one serial chain) and each is
+ * executed wave-wide, literals then match, straight into the frame's
+ * output.
+ *
+ * The content checksum of a frame is checked by a second kernel, four
+ * lanes a frame (xxhash.h), over every frame that decoded and carries one:
+ * one frame's XXH64 is a serial chain of stripes, so frames in parallel is
+ * what fills the device. */
+#include <mutex>
+
+#include "common.h"
+#include "cudec_zstd.h"
+#include "xxhash.h"
+#include "zstd_decode.h"
+
+namespace cudec_amd {
+namespace {
+
+constexpr uint32_t kFramesPerCu = 8;     /* waves of the persistent grid per CU */
+constexpr int kVerifyBlock = 256;
+constexpr uint64_t kPoolKeep = 512ull << 20; /* scratch kept between launches */
+
+struct Wave {
+    static constexpr uint32_t kLanes = kWave;
+    uint32_t lane;
+    __device__ void sync() const { wave_order(); }
+};
+
+/* What the decode leaves for the checksum pass, per frame. */
+struct FrameCheck {
+    uint64_t written;
+    uint32_t checksum;
+    uint32_t verify; /* 1: the frame decoded and carries a content checksum */
+};
+
+/* Scratch of one launch: the frame counter, then a FrameCheck per frame,
+ * then a literals slot of kBlockCeiling bytes per wave of the grid. */
+constexpr size_t kCounterBytes = 256;
+
+__global__ void zstd_reset(unsigned long long* counter) {
+    if (threadIdx.x == 0) {
+        *counter = 0;
+    }
+}
+
+__global__ void __launch_bounds__(kWave)
+    zstd_batch(const void* const* srcs, const size_t* sizes, void* const* dsts, const size_t* caps,
+               size_t n, cudec_chunk_result* results, unsigned long long* counter,
+               FrameCheck* checks, uint8_t* lits_base) {
+    __shared__ zstd::FrameTables tables;
+    const Wave w{__lane_id()};
+    uint8_t* lits = lits_base + (size_t)blockIdx.x * zstd::kBlockCeiling;
+    /* The loop body has no lane-conditional region: every lane issues the
+     * atomic (lane 0 adds one, the others zero, so lane 0's old value is
+     * the frame's index whatever order the lanes are applied in) and every
+     * lane stores the same result. A lane-0-only atomic here let the
+     * compiler's structurizer send lanes 1..63 back to the index read
+     * without lane 0, which then read index 0 for ever. */
+    for (;;) {
+        const unsigned long long old = atomicAdd(counter, w.lane == 0 ? 1ull : 0ull);
+        const uint64_t i = uniform64(old);
+        if (i >= n) {
+            break;
+        }
+        const uint8_t* src = uniform_ptr(static_cast<const uint8_t*>(srcs[i]));
+        uint8_t* dst = uniform_ptr(static_cast<uint8_t*>(dsts[i]));
+        const uint64_t size = uniform64(sizes[i]);
+        const uint64_t cap = uniform64(caps[i]);
+        zstd::FrameEnd end{0, 0, false};
+        const int st = (int)uniform32((uint32_t)zstd::decode_frame(w, src, size, dst, cap, tables,
+                                                                     lits, end));
+        write_result(results, i, st, st == CUDEC_OK ? uniform64(end.written) : 0);
+        FrameCheck c;
+        c.written = uniform64(end.written);
+        c.checksum = uniform32(end.checksum);
+        c.verify = st == CUDEC_OK && end.has_checksum ? 1u : 0u;
+        checks[i] = c;
+        wave_order();
+    }
+}
+
+/* Frames that decoded and carry a content checksum: the low 32 bits of the
+ * XXH64 (seed 0) of the output must equal it (RFC 8878 3.1.1). */
+__global__ void __launch_bounds__(kVerifyBlock)
+    zstd_verify(void* const* dsts, size_t n, const FrameCheck* checks,
+                cudec_chunk_result* results) {
+    const uint32_t lane = __lane_id();
+    const uint32_t q = lane % xxh::kGroup;
+    constexpr uint32_t kPerBlock = kVerifyBlock / xxh::kGroup;
+    const size_t per_grid = (size_t)gridDim.x * kPerBlock;
+    for (size_t i = (size_t)blockIdx.x * kPerBlock + threadIdx.x / xxh::kGroup;; i += per_grid) {
+        const bool in_range = i < n;
+        if (__ballot(in_range) == 0) {
+            break;
+        }
+        const bool live = in_range && checks[i].verify != 0;
+        const uint64_t h = xxh::xxh64_group(live ? (gcu8*)dsts[i] : nullptr,
+                                            live ? checks[i].written : 0, 0, q, lane - q);
+        if (live && q == 0 && (uint32_t)h != checks[i].checksum) {
+            write_result(results, i, CUDEC_ERR_CORRUPT_INPUT, 0);
+        }
+    }
+}
+
+std::mutex g_pool_lock;
+constexpr int kMaxDevices = 64;
+hipMemPool_t g_pool[kMaxDevices];
+bool g_pool_tried[kMaxDevices];
+
+/* The device's scratch pool for this decoder, made on first use; it keeps
+ * up to kPoolKeep bytes of freed scratch between launches. */
+hipMemPool_t zstd_pool(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> hold(g_pool_lock);
+    if (!g_pool_tried[dev]) {
+        g_pool_tried[dev] = true;
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) == hipSuccess) {
+            uint64_t keep = kPoolKeep;
+            if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) {
+                g_pool[dev] = pool;
+            } else {
+                (void)hipMemPoolDestroy(pool);
+            }
+        }
+        (void)hipGetLastError(); /* a pool that could not be made is not an error */
+    }
+    return g_pool[dev];
+}
+
+}  // namespace
+
+hipError_t launch_zstd(const void* const* srcs, const size_t* sizes, void* const* dsts,
+                       const size_t* caps, size_t n, cudec_chunk_result* results,
+                       hipStream_t stream) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) {
+        return e;
+    }
+    const size_t waves = (size_t)(cus > 0 ? cus : 1) * kFramesPerCu;
+    const size_t grid = n < waves ? n : waves;
+    const size_t checks_bytes = (sizeof(FrameCheck) * n + 255) & ~(size_t)255;
+    const size_t bytes = kCounterBytes + checks_bytes + grid * zstd::kBlockCeiling;
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    e = hipStreamIsCapturing(stream, &capture);
+    if (e != hipSuccess) {
+        return e;
+    }
+    hipMemPool_t pool = capture == hipStreamCaptureStatusNone ? zstd_pool(dev) : nullptr;
+    void* mem = nullptr;
+    e = pool != nullptr ? hipMallocFromPoolAsync(&mem, bytes, pool, stream)
+                        : hipMallocAsync(&mem, bytes, stream);
+    if (e != hipSuccess) {
+        return e;
+    }
+    uint8_t* base = static_cast<uint8_t*>(mem);
+    auto* counter = reinterpret_cast<unsigned long long*>(base);
+    auto* checks = reinterpret_cast<FrameCheck*>(base + kCounterBytes);
+    uint8_t* lits = base + kCounterBytes + checks_bytes;
+    hipLaunchKernelGGL(zstd_reset, dim3(1), dim3(kWave), 0, stream, counter);
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(zstd_batch, dim3((unsigned)grid), dim3(kWave), 0, stream, srcs, sizes,
+                           dsts, caps, n, results, counter, checks, lits);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        constexpr size_t kPerBlock = kVerifyBlock / xxh::kGroup;
+        size_t vgrid = (n + kPerBlock - 1) / kPerBlock;
+        vgrid = vgrid > 8192 ? 8192 : vgrid;
+        hipLaunchKernelGGL(zstd_verify, dim3((unsigned)vgrid), dim3(kVerifyBlock), 0, stream, dsts,
+                           n, checks, results);
+        e = hipGetLastError();
+    }
+    const hipError_t f = hipFreeAsync(mem, stream);
+    return e != hipSuccess ? e : f;
+}
+
+}  // namespace cudec_amd

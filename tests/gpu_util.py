@@ -21,6 +21,8 @@ def device_available() -> bool:
 def oracle_decode(fmt: str, stream: bytes, cap: int):
     if fmt == "lz4":
         return libs.oracle_lz4(stream, cap)
+    if fmt == "zstd":
+        return libs.ref_zstd_decode(stream, cap)
     st, out, _ = libs.oracle_snappy(stream, cap)
     return st, out
 

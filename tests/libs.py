@@ -381,3 +381,90 @@ def xxhash_lib():
         lib.XXH64.restype = ctypes.c_uint64
         return lib
     return None
+
+
+# ------------------------------------------------------------------ zstd
+
+ZSTD_C_LEVEL, ZSTD_C_WINDOW_LOG = 100, 101
+ZSTD_C_CONTENT_SIZE, ZSTD_C_CHECKSUM = 200, 201
+ZSTD_C_LITERAL_MODE, ZSTD_C_TARGET_BLOCK = 1002, 1003  # experimental in 1.4.9
+UNSUPPORTED = 6
+
+
+@lru_cache(None)
+def zstdlib():
+    """libzstd (1.4.9 in this image), the compressor of the Zstd corpora and
+    the byte oracle of valid frames; None where it is absent."""
+    lib = _first(["/opt/conda/lib/libzstd.so.1", "libzstd.so.1"])
+    if lib is None:
+        return None
+    lib.ZSTD_versionString.restype = ctypes.c_char_p
+    lib.ZSTD_compressBound.argtypes = [ctypes.c_size_t]
+    lib.ZSTD_compressBound.restype = ctypes.c_size_t
+    lib.ZSTD_isError.argtypes = [ctypes.c_size_t]
+    lib.ZSTD_isError.restype = ctypes.c_uint
+    lib.ZSTD_createCCtx.restype = ctypes.c_void_p
+    lib.ZSTD_freeCCtx.argtypes = [ctypes.c_void_p]
+    lib.ZSTD_CCtx_setParameter.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.ZSTD_CCtx_setParameter.restype = ctypes.c_size_t
+    lib.ZSTD_compress2.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                   ctypes.c_char_p, ctypes.c_size_t]
+    lib.ZSTD_compress2.restype = ctypes.c_size_t
+    lib.ZSTD_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p,
+                                    ctypes.c_size_t]
+    lib.ZSTD_decompress.restype = ctypes.c_size_t
+    return lib
+
+
+def zstd_compress(data: bytes, level: int = 3, **params) -> bytes:
+    """One frame of `data` through ZSTD_compress2 with the given parameters
+    (ids above, by keyword: window_log, content_size, checksum,
+    literal_mode, target_block)."""
+    z = zstdlib()
+    ids = {"window_log": ZSTD_C_WINDOW_LOG, "content_size": ZSTD_C_CONTENT_SIZE,
+           "checksum": ZSTD_C_CHECKSUM, "literal_mode": ZSTD_C_LITERAL_MODE,
+           "target_block": ZSTD_C_TARGET_BLOCK}
+    cctx = z.ZSTD_createCCtx()
+    try:
+        for pid, val in [(ZSTD_C_LEVEL, level)] + [(ids[k], v) for k, v in params.items()]:
+            r = z.ZSTD_CCtx_setParameter(cctx, pid, int(val))
+            assert not z.ZSTD_isError(r), f"libzstd refused parameter {pid}={val}"
+        cap = z.ZSTD_compressBound(len(data)) + 64
+        buf = ctypes.create_string_buffer(cap)
+        n = z.ZSTD_compress2(cctx, buf, cap, data, len(data))
+        assert not z.ZSTD_isError(n)
+        return buf.raw[:n]
+    finally:
+        z.ZSTD_freeCCtx(cctx)
+
+
+def zstd_decompress(frame: bytes, capacity: int):
+    """(ok, bytes) from libzstd's ZSTD_decompress."""
+    z = zstdlib()
+    buf = ctypes.create_string_buffer(max(capacity, 1))
+    n = z.ZSTD_decompress(buf, capacity, frame, len(frame))
+    if z.ZSTD_isError(n):
+        return False, b""
+    return True, buf.raw[:n]
+
+
+@lru_cache(None)
+def ref_zstd():
+    """The reference's Zstd frame twin compiled from the reference tree
+    (oracle/_ref/libcudec_ref_zstd.so), or None where that build is absent."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libcudec_ref_zstd.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.ref_zstd_decode.argtypes = [ctypes.c_char_p, u64, ctypes.c_void_p, u64, u64p, i32p]
+    lib.ref_zstd_decode.restype = i32
+    return lib
+
+
+def ref_zstd_decode(frame: bytes, capacity: int):
+    """(status, output) of one frame through the reference's twin."""
+    buf = ctypes.create_string_buffer(max(capacity, 1))
+    w, stage = u64(0), i32(0)
+    st = ref_zstd().ref_zstd_decode(frame, len(frame), buf, capacity, ctypes.byref(w),
+                                    ctypes.byref(stage))
+    return st, (buf.raw[:w.value] if st == OK else b"")

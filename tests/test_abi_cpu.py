@@ -61,6 +61,7 @@ def test_headers_compile_and_link(tmp_path, std, cc):
     src.write_text(textwrap.dedent("""
         #include "cudec.h"
         #include "cudec_snappy.h"
+        #include "cudec_zstd.h"
         #include "cudec_mi355x.h"
         #include <stdio.h>
         int main(void) {
@@ -75,6 +76,7 @@ def test_headers_compile_and_link(tmp_path, std, cc):
             if (cudec_lz4_decompress_stream(0, 0, 0, 0, 1, CUDEC_MEM_HOST, 0) != CUDEC_ERR_INVALID_ARGUMENT) return 7;
             cudec_stream_ctx_destroy(0);
             if (cudec_mi355x_max_batch_chunks() != (size_t)0x7fffffff * 64) return 8;
+            if (cudec_zstd_decompress_batch(0, 0, 0, 0, 1, 0, 0) != CUDEC_ERR_INVALID_ARGUMENT) return 9;
             puts("ok");
             return 0;
         }
