@@ -12,7 +12,9 @@ tuned form (cudec_lz4_decompress_batch) is the measured value; the plain
 one-wave-per-chunk form (cudec_lz4_decompress_batch_plain) is timed on the
 same batch for the "vs plain" ratio. Every chunk of every form is verified
 byte-exact against its original (and status OK, bytes_written == size)
-before timing.
+before timing. --format snappy and --format zstd run the same batch as raw
+Snappy blocks and as Zstandard frames (libzstd level 3, one frame a chunk,
+cudec_zstd_decompress_batch).
 
 Output: one JSON line (see the keys at the bottom of main()).
 """
@@ -42,7 +44,7 @@ def parse_args():
     p.add_argument("--steps", type=int, default=30)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", choices=["uneven", "token64k"], default="uneven")
-    p.add_argument("--format", choices=["lz4", "snappy"], default="lz4")
+    p.add_argument("--format", choices=["lz4", "snappy", "zstd"], default="lz4")
     p.add_argument("--chunks", type=int, default=None)
     p.add_argument("--seed", type=int, default=0x5EED)
     p.add_argument("--forms", default="tuned,plain")
@@ -81,7 +83,8 @@ def parse_args():
 
 def build_batch(args):
     from cudec_amd import workload
-    codec = workload.CODEC_LZ4 if args.format == "lz4" else workload.CODEC_SNAPPY
+    codec = {"lz4": workload.CODEC_LZ4, "snappy": workload.CODEC_SNAPPY,
+             "zstd": workload.CODEC_ZSTD}[args.format]
     t0 = time.perf_counter()
     if args.workload == "uneven":
         n = args.chunks or 20000
@@ -100,6 +103,12 @@ def build_batch(args):
     if len(malformed):
         name += f", {pct:g}% malformed ({len(malformed)} chunks: truncation or byte flip)"
     return b, name, malformed
+
+
+def workload_codec(fmt: str) -> str:
+    from cudec_amd import workload
+    return workload.codec_version({"lz4": workload.CODEC_LZ4, "snappy": workload.CODEC_SNAPPY,
+                                   "zstd": workload.CODEC_ZSTD}[fmt])
 
 
 def malform(b, pct, seed):
@@ -251,7 +260,7 @@ def cpu_model() -> str:
 # worker threads than that, whatever the host shows.
 CPU_SHARE = 16
 
-DEC_LIBLZ4, DEC_LIBSNAPPY, DEC_TWIN_LZ4, DEC_TWIN_SNAPPY = 0, 1, 2, 3
+DEC_LIBLZ4, DEC_LIBSNAPPY, DEC_TWIN_LZ4, DEC_TWIN_SNAPPY, DEC_LIBZSTD, DEC_TWIN_ZSTD = 0, 1, 2, 3, 4, 5
 
 
 def cpu_threads(requested=None) -> tuple[int, int]:
@@ -265,8 +274,12 @@ def twin_decoders(fmt):
     """The reference's host ladders compiled from its source (oracle/_ref,
     kind "reference") where that build is present, else the oracle's C
     restatement (kind "port"): (function address, kind, name)."""
-    ref_path = os.path.join(ROOT, "oracle", "_ref", "libcudec_ref.so")
+    ref_path = os.path.join(ROOT, "oracle", "_ref",
+                            "libcudec_ref_zstd.so" if fmt == "zstd" else "libcudec_ref.so")
     port_path = os.path.join(ROOT, "oracle", "liboracle.so")
+    if fmt == "zstd" and not os.path.exists(ref_path):
+        raise SystemExit("[bench] the Zstd CPU baseline needs oracle/_ref/libcudec_ref_zstd.so "
+                         "(the reference's frame twin, built from the reference tree)")
     if os.path.exists(ref_path):
         lib, kind, prefix = ctypes.CDLL(ref_path), "reference", "ref"
     else:
@@ -342,11 +355,12 @@ def cpu_baseline(batch, fmt, threads, valid, visible):
     import numpy as np
     from cudec_amd import workload
     fn, kind, twin_name, _keep = twin_decoders(fmt)
-    twin = DEC_TWIN_LZ4 if fmt == "lz4" else DEC_TWIN_SNAPPY
-    lib_kind = DEC_LIBLZ4 if fmt == "lz4" else DEC_LIBSNAPPY
-    codec = workload.CODEC_LZ4 if fmt == "lz4" else workload.CODEC_SNAPPY
-    lib_name = ("LZ4_decompress_safe" if fmt == "lz4" else "snappy_uncompress") + \
-        f" ({workload.codec_version(codec)})"
+    twin = {"lz4": DEC_TWIN_LZ4, "snappy": DEC_TWIN_SNAPPY, "zstd": DEC_TWIN_ZSTD}[fmt]
+    lib_kind = {"lz4": DEC_LIBLZ4, "snappy": DEC_LIBSNAPPY, "zstd": DEC_LIBZSTD}[fmt]
+    codec = {"lz4": workload.CODEC_LZ4, "snappy": workload.CODEC_SNAPPY,
+             "zstd": workload.CODEC_ZSTD}[fmt]
+    lib_name = {"lz4": "LZ4_decompress_safe", "snappy": "snappy_uncompress",
+                "zstd": "ZSTD_decompress"}[fmt] + f" ({workload.codec_version(codec)})"
     full = CpuBatch(batch)
     rows = []
     row, status, written = cpu_row(full, twin, fn, threads, f"twin {twin_name} [{kind}]")
@@ -578,6 +592,10 @@ def main():
     uncomp = batch.total_uncompressed
     algo = uncomp + batch.total_compressed
     forms = [f for f in args.forms.split(",") if f]
+    if args.format == "zstd":
+        # one launch form, and no decode plan, parse-only build or graph leg
+        forms = ["tuned"]
+        args.no_autotune = args.no_parse = args.no_graph = True
     default_config = (args.workload == "uneven" and args.format == "lz4" and args.chunks is None
                       and args.seed == 0x5EED and forms[0] == "tuned")
     measured = {}
@@ -588,7 +606,7 @@ def main():
         if cudec_amd.set_pipeline_threshold(args.format, t) != 0:
             raise SystemExit("[bench] could not set the pipelining threshold")
         log(f"pipelining threshold set to {t}")
-    elif "tuned" in forms and not args.no_autotune:
+    elif "tuned" in forms and not args.no_autotune and args.format != "zstd":
         import cudec_amd
         t0 = time.perf_counter()
         st, lanes, ms_threshold, ms_plan = cudec_amd.autotune_plan(
@@ -669,8 +687,10 @@ def main():
     elif default_config:
         traffic = committed_traffic()
     line = {
-        "metric": "LZ4 batch decode GB/s (uncompressed) + % HBM peak, vs plain; p99 request latency"
-        if args.format == "lz4" else "Snappy batch decode GB/s (uncompressed) + % HBM peak, vs plain",
+        "metric": {"lz4": "LZ4 batch decode GB/s (uncompressed) + % HBM peak, vs plain; "
+                          "p99 request latency",
+                   "snappy": "Snappy batch decode GB/s (uncompressed) + % HBM peak, vs plain",
+                   "zstd": "Zstd frame batch decode GB/s (uncompressed) + % HBM peak"}[args.format],
         "value": round(head["gbps"], 3),
         "unit": "GB/s (uncompressed)",
         "n_gpus": 1,
@@ -681,7 +701,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
-        "data": "synthetic (seeded generator, liblz4/snappy-compressed on the host)",
+        "data": "synthetic (seeded generator, compressed on the host with " +
+                workload_codec(args.format) + ")",
         "config": {"workload": workload_name, "chunks": batch.n, "form": forms[0],
                    "uncompressed_bytes": uncomp, "compressed_bytes": batch.total_compressed,
                    "seed": args.seed},

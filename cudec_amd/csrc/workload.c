@@ -7,9 +7,10 @@
  * Zipf and log-uniform draws) so the same seed yields the same bytes on any
  * host; no libm call sits on a path that decides a byte.
  *
- * The compressors are liblz4's LZ4_compress_default (acceleration 1) and
- * snappy's C API (snappy_compress == snappy::RawCompress), loaded with dlopen
- * so the library builds and loads on a host without them; wl_codec_version
+ * The compressors are liblz4's LZ4_compress_default (acceleration 1),
+ * snappy's C API (snappy_compress == snappy::RawCompress) and libzstd's
+ * ZSTD_compress (level 3, one frame per chunk), loaded with dlopen so the
+ * library builds and loads on a host without them; wl_codec_version
  * reports what was found. */
 #define _GNU_SOURCE
 #include <dlfcn.h>
@@ -396,7 +397,8 @@ void wl_gen_batch(uint8_t* arena, const uint64_t* offsets,
 
 /* ------------------------------------------------------- compressors */
 
-enum { WL_CODEC_LZ4 = 0, WL_CODEC_SNAPPY = 1 };
+enum { WL_CODEC_LZ4 = 0, WL_CODEC_SNAPPY = 1, WL_CODEC_ZSTD = 2 };
+#define WL_ZSTD_LEVEL 3
 
 typedef int (*lz4_compress_fn)(const char*, char*, int, int);
 typedef int (*lz4_decompress_fn)(const char*, char*, int, int);
@@ -405,6 +407,11 @@ typedef const char* (*lz4_version_fn)(void);
 typedef int (*snappy_compress_fn)(const char*, size_t, char*, size_t*);
 typedef int (*snappy_uncompress_fn)(const char*, size_t, char*, size_t*);
 typedef size_t (*snappy_max_fn)(size_t);
+typedef size_t (*zstd_compress_fn)(void*, size_t, const void*, size_t, int);
+typedef size_t (*zstd_decompress_fn)(void*, size_t, const void*, size_t);
+typedef size_t (*zstd_bound_fn)(size_t);
+typedef unsigned (*zstd_is_error_fn)(size_t);
+typedef const char* (*zstd_version_fn)(void);
 
 static struct {
     void* lz4;
@@ -417,6 +424,12 @@ static struct {
     snappy_uncompress_fn snappy_uncompress;
     snappy_max_fn snappy_max;
     char snappy_path[512];
+    void* zstd;
+    zstd_compress_fn zstd_compress;
+    zstd_decompress_fn zstd_decompress;
+    zstd_bound_fn zstd_bound;
+    zstd_is_error_fn zstd_is_error;
+    zstd_version_fn zstd_version;
 } g_codecs;
 static pthread_once_t g_codecs_once = PTHREAD_ONCE_INIT;
 
@@ -465,6 +478,17 @@ static void load_codecs(void) {
             free(real);
         }
     }
+    static const char* const zstd_names[] = {
+        "/opt/conda/lib/libzstd.so.1", "libzstd.so.1",
+        "/usr/lib/x86_64-linux-gnu/libzstd.so.1", NULL};
+    g_codecs.zstd = open_first(zstd_names);
+    if (g_codecs.zstd) {
+        g_codecs.zstd_compress = (zstd_compress_fn)dlsym(g_codecs.zstd, "ZSTD_compress");
+        g_codecs.zstd_decompress = (zstd_decompress_fn)dlsym(g_codecs.zstd, "ZSTD_decompress");
+        g_codecs.zstd_bound = (zstd_bound_fn)dlsym(g_codecs.zstd, "ZSTD_compressBound");
+        g_codecs.zstd_is_error = (zstd_is_error_fn)dlsym(g_codecs.zstd, "ZSTD_isError");
+        g_codecs.zstd_version = (zstd_version_fn)dlsym(g_codecs.zstd, "ZSTD_versionString");
+    }
 }
 
 int32_t wl_codec_available(int32_t codec) {
@@ -477,13 +501,18 @@ int32_t wl_codec_available(int32_t codec) {
         return g_codecs.snappy_compress && g_codecs.snappy_uncompress &&
                g_codecs.snappy_max;
     }
+    if (codec == WL_CODEC_ZSTD) {
+        return g_codecs.zstd_compress && g_codecs.zstd_decompress && g_codecs.zstd_bound &&
+               g_codecs.zstd_is_error;
+    }
     return 0;
 }
 
 /* A human-readable name of the library found ("liblz4 1.9.3",
- * "libsnappy (/opt/conda/lib/libsnappy.so.1)") or "absent". */
+ * "libsnappy (/opt/conda/lib/libsnappy.so.1)", "libzstd 1.4.9 level 3") or
+ * "absent". */
 const char* wl_codec_version(int32_t codec) {
-    static char buf[2][600];
+    static char buf[3][600];
     pthread_once(&g_codecs_once, load_codecs);
     if (codec == WL_CODEC_LZ4) {
         if (!g_codecs.lz4_compress) {
@@ -492,6 +521,14 @@ const char* wl_codec_version(int32_t codec) {
         snprintf(buf[0], sizeof buf[0], "liblz4 %s",
                  g_codecs.lz4_version ? g_codecs.lz4_version() : "?");
         return buf[0];
+    }
+    if (codec == WL_CODEC_ZSTD) {
+        if (!g_codecs.zstd_compress) {
+            return "absent";
+        }
+        snprintf(buf[2], sizeof buf[2], "libzstd %s level %d",
+                 g_codecs.zstd_version ? g_codecs.zstd_version() : "?", WL_ZSTD_LEVEL);
+        return buf[2];
     }
     if (!g_codecs.snappy_compress) {
         return "absent";
@@ -507,6 +544,9 @@ uint64_t wl_compress_bound(int32_t codec, uint64_t n) {
     }
     if (codec == WL_CODEC_SNAPPY && g_codecs.snappy_max) {
         return (uint64_t)g_codecs.snappy_max((size_t)n);
+    }
+    if (codec == WL_CODEC_ZSTD && g_codecs.zstd_bound) {
+        return (uint64_t)g_codecs.zstd_bound((size_t)n);
     }
     return 0;
 }
@@ -532,6 +572,10 @@ uint64_t wl_compress(int32_t codec, const uint8_t* src, uint64_t n,
                                         (char*)dst, &out) == 0
                    ? (uint64_t)out
                    : 0;
+    }
+    if (codec == WL_CODEC_ZSTD && g_codecs.zstd_compress) {
+        const size_t w = g_codecs.zstd_compress(dst, (size_t)cap, src, (size_t)n, WL_ZSTD_LEVEL);
+        return g_codecs.zstd_is_error(w) ? 0 : (uint64_t)w;
     }
     return 0;
 }
@@ -561,6 +605,14 @@ int32_t wl_decompress(int32_t codec, const uint8_t* src, uint64_t n,
             return 0;
         }
         *produced = (uint64_t)out;
+        return 1;
+    }
+    if (codec == WL_CODEC_ZSTD && g_codecs.zstd_decompress) {
+        const size_t w = g_codecs.zstd_decompress(dst, (size_t)cap, src, (size_t)n);
+        if (g_codecs.zstd_is_error(w)) {
+            return 0;
+        }
+        *produced = (uint64_t)w;
         return 1;
     }
     return 0;
@@ -629,6 +681,8 @@ enum {
     WL_DEC_LIBSNAPPY = 1,   /* snappy_uncompress (capacity checked first) */
     WL_DEC_TWIN_LZ4 = 2,    /* int32 fn(src, size, dst, cap, uint64* written) */
     WL_DEC_TWIN_SNAPPY = 3, /* int32 fn(src, size, dst, cap, uint64* written, int32* rung) */
+    WL_DEC_LIBZSTD = 4,     /* ZSTD_decompress (one frame per chunk) */
+    WL_DEC_TWIN_ZSTD = 5,   /* int32 fn(src, size, dst, cap, uint64* written, int32* stage) */
 };
 typedef int32_t (*twin_lz4_fn)(const uint8_t*, uint64_t, uint8_t*, uint64_t, uint64_t*);
 typedef int32_t (*twin_snappy_fn)(const uint8_t*, uint64_t, uint8_t*, uint64_t, uint64_t*,
@@ -680,6 +734,13 @@ static void decode_part(time_part* t) {
             case WL_DEC_TWIN_LZ4:
                 st = ((twin_lz4_fn)t->fn)(t->srcs[i], t->sizes[i], t->dsts[i], t->caps[i], &w);
                 break;
+            case WL_DEC_LIBZSTD: {
+                const size_t r = g_codecs.zstd_decompress(t->dsts[i], (size_t)t->caps[i],
+                                                          t->srcs[i], (size_t)t->sizes[i]);
+                st = g_codecs.zstd_is_error(r) ? 1 : 0;
+                w = st ? 0 : (uint64_t)r;
+                break;
+            }
             default: {
                 int32_t rung = 0;
                 st = ((twin_snappy_fn)t->fn)(t->srcs[i], t->sizes[i], t->dsts[i], t->caps[i], &w,
@@ -742,7 +803,11 @@ int32_t wl_time_decode(int32_t kind, void* fn, const uint8_t* const* srcs,
             return -1;
         }
     }
-    if ((kind == WL_DEC_TWIN_LZ4 || kind == WL_DEC_TWIN_SNAPPY) && !fn) {
+    if (kind == WL_DEC_LIBZSTD && (!g_codecs.zstd_decompress || !g_codecs.zstd_is_error)) {
+        return -1;
+    }
+    if ((kind == WL_DEC_TWIN_LZ4 || kind == WL_DEC_TWIN_SNAPPY || kind == WL_DEC_TWIN_ZSTD) &&
+        !fn) {
         return -1;
     }
     if (threads < 1) {

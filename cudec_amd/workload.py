@@ -27,7 +27,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 _WL = None
 
 KIND_TOKENS, KIND_RANDOM, KIND_REPETITIVE = 0, 1, 2
-CODEC_LZ4, CODEC_SNAPPY = 0, 1
+CODEC_LZ4, CODEC_SNAPPY, CODEC_ZSTD = 0, 1, 2
 
 
 def wl() -> ctypes.CDLL:
