@@ -120,6 +120,9 @@ cudec_status cudec_zstd_decompress_batch(const void* const* d_src_ptrs,
     if (valid != CUDEC_OK) {
         return valid;
     }
+    if (chunk_count > UINT32_MAX) {
+        return CUDEC_ERR_INVALID_ARGUMENT; /* frames are ordered by 32-bit index */
+    }
     (void)hipGetLastError();
     const hipError_t err = launch_zstd(d_src_ptrs, d_src_sizes, d_dst_ptrs, d_dst_capacities,
                                        chunk_count, d_results,

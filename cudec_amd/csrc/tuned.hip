@@ -2368,6 +2368,33 @@ hipError_t queue_and_decode(DeviceState* dev, int format, const void* const* src
 
 }  // namespace
 
+size_t cost_order_bytes(size_t n) {
+    return sizeof(Queue) + sizeof(uint32_t) * n;
+}
+
+/* The order of a batch by decode cost, costliest first, for a decoder
+ * outside this kernel family (the Zstandard frames, whose time also follows
+ * the output plus what matches add): the same queue build with every chunk
+ * in one class. `work` holds cost_order_bytes(n) bytes; *order points into
+ * it (u32 chunk indices, n <= UINT32_MAX). */
+hipError_t cost_order(const size_t* sizes, const size_t* caps, size_t n, void* work,
+                      hipStream_t stream, const uint32_t** order) {
+    if (n > UINT32_MAX) {
+        return hipErrorInvalidValue;
+    }
+    const Plan plan{};
+    Queue* q = static_cast<Queue*>(work);
+    uint32_t* out = reinterpret_cast<uint32_t*>(q + 1);
+    const unsigned tb = 256;
+    const unsigned nb = (unsigned)((n + tb - 1) / tb);
+    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(kKeys), 0, stream, q);
+    hipLaunchKernelGGL(queue_count, dim3(nb), dim3(tb), 0, stream, sizes, caps, n, q, plan);
+    hipLaunchKernelGGL(queue_scan, dim3(1), dim3(1), 0, stream, q);
+    hipLaunchKernelGGL(queue_fill, dim3(nb), dim3(tb), 0, stream, sizes, caps, n, q, out, plan);
+    *order = out;
+    return hipGetLastError();
+}
+
 hipError_t launch_tuned(int format, const void* const* srcs, const size_t* src_sizes,
                         void* const* dsts, const size_t* caps, size_t n,
                         cudec_chunk_result* results, hipStream_t stream, bool parse_only) {

@@ -9,7 +9,8 @@
  * survive block boundaries: a Treeless literals section reuses the last
  * Huffman table and a Repeat sequence mode the last FSE table of its field.
  *
- * What runs in parallel is given by the Wave policy (kLanes, lane, sync): a
+ * What runs in parallel is given by the Wave policy (kLanes, lane, sync,
+ * uniform: a value every lane holds, as one register for all of them): a
  * wave of 64 lanes on the device, one lane on the host. The serial parts run
  * on every lane with the same values (uniform code); the lane-parallel parts
  * are the four Huffman streams of a literals section, the Huffman table
@@ -39,6 +40,19 @@
 #define ZD __host__ __device__ __forceinline__
 #else
 #define ZD inline
+#endif
+
+/* Address spaces of the device build: ZG marks global memory (the frames,
+ * the output, the literals scratch), ZL the wave's LDS (tables, windows).
+ * Named spaces give global_* and ds_* instructions; through generic
+ * pointers every access would be a flat one, which waits for outstanding
+ * global stores and LDS reads alike. Empty in the CPU build. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ZG __attribute__((address_space(1)))
+#define ZL __attribute__((address_space(3)))
+#else
+#define ZG
+#define ZL
 #endif
 
 namespace cudec_amd {
@@ -85,7 +99,7 @@ constexpr int16_t kMatchLenDefault[53] = {1, 4, 3, 2, 2, 2, 2, 2, 2, 1, 1, 1, 1,
 
 ZD uint32_t high_bit(uint32_t v) { return v == 0 ? 0u : 31u - (uint32_t)__builtin_clz(v); }
 
-ZD uint64_t read_le(const uint8_t* p, uint32_t n) {
+ZD uint64_t read_le(const ZG uint8_t* p, uint32_t n) {
     uint64_t v = 0;
     for (uint32_t i = 0; i < n; i++) {
         v |= (uint64_t)p[i] << (8 * i);
@@ -99,15 +113,32 @@ ZD uint64_t read_le(const uint8_t* p, uint32_t n) {
  * first, starting below the highest set bit of the last byte. `acc` holds
  * the next `n` bits in its low bits, the next bit highest; bytes [0, pos)
  * are not loaded yet. */
-struct BackBits {
-    const uint8_t* p;
+/* How a reader's loaded words are held: PerLane keeps each lane's own
+ * (a lane decoding its own stream); the Wave policy's Same makes them one
+ * wave-uniform value (every lane decoding the same stream), so the
+ * reader's state lives in scalar registers and its tests are scalar
+ * branches. */
+struct PerLane {
+    static ZD uint64_t fix64(uint64_t v) { return v; }
+};
+
+template <class U>
+struct BackBitsT {
+    const ZG uint8_t* p;
     uint64_t pos;
     uint64_t acc;
     uint32_t n;
+    /* An optional copy of stream bytes [win_lo, win_hi) in faster memory
+     * (LDS on the device), which refills read instead of p when it holds
+     * the bytes they need. */
+    const ZL uint8_t* win;
+    uint64_t win_lo, win_hi;
 
     /* False for an empty stream or a last byte of zero (no start marker). */
-    ZD bool start(const uint8_t* s, uint64_t size) {
+    ZD bool start(const ZG uint8_t* s, uint64_t size) {
         p = s;
+        win = nullptr;
+        win_lo = win_hi = 0;
         if (size == 0 || s[size - 1] == 0) {
             pos = 0;
             acc = 0;
@@ -122,11 +153,67 @@ struct BackBits {
     }
     ZD uint64_t remaining() const { return n + 8 * pos; }
     ZD bool at_end() const { return n == 0 && pos == 0; }
+    /* Tops the accumulator up to at least 57 bits (or to the stream's
+     * start): one 8-byte load takes the next k <= 7 bytes, so a refill costs
+     * one memory round trip, not one per byte. */
     ZD void refill() {
+        if (n > 56) {
+            return;
+        }
+        if (pos >= 8) {
+            uint32_t k = (64 - n) >> 3;
+            k = k < 7 ? k : 7;
+            uint64_t word = 0; /* bytes pos - 8 .. pos - 1, last on top */
+            if (win != nullptr && pos - 8 >= win_lo && pos <= win_hi) {
+                const ZL uint8_t* from = win + (pos - 8 - win_lo);
+                for (uint32_t j = 0; j < 8; j++) { /* bytes: windows are unaligned */
+                    word |= (uint64_t)from[j] << (8 * j);
+                }
+            } else {
+                __builtin_memcpy(&word, (const void*)(p + pos - 8), 8);
+            }
+            word = U::fix64(word);
+            acc = (acc << (8 * k)) | (word >> (64 - 8 * k));
+            pos -= k;
+            n += 8 * k;
+            return;
+        }
         while (n <= 56 && pos > 0) {
-            acc = (acc << 8) | p[--pos];
+            pos--;
+            const uint64_t b =
+                win != nullptr && pos >= win_lo && pos < win_hi ? win[pos - win_lo] : p[pos];
+            acc = (acc << 8) | U::fix64(b);
             n += 8;
         }
+    }
+    /* Whether the next refill would read bytes the window does not hold. */
+    ZD bool needs_stage() const {
+        if (n > 56 || pos == 0) {
+            return false;
+        }
+        const bool covered = win != nullptr && pos <= win_hi &&
+                             (pos >= 8 ? pos - 8 >= win_lo : win_lo == 0);
+        return !covered;
+    }
+    /* Copies stream bytes [pos - cap, pos) (from 0 when pos < cap) into
+     * buf and reads from there; one lane, 64 bytes of loads in flight. */
+    ZD void stage_lane(ZL uint8_t* buf, uint32_t cap) {
+        const uint64_t lo = pos > cap ? pos - cap : 0;
+        const uint32_t len = (uint32_t)(pos - lo);
+        uint32_t i = 0;
+        for (; i + 64 <= len; i += 64) {
+            uint8_t v[64];
+            __builtin_memcpy(v, (const void*)(p + lo + i), 64);
+            for (uint32_t j = 0; j < 64; j++) {
+                buf[i + j] = v[j];
+            }
+        }
+        for (; i < len; i++) {
+            buf[i] = p[lo + i];
+        }
+        win = buf;
+        win_lo = lo;
+        win_hi = pos;
     }
     /* The next c (<= 32) bits; false, reading nothing, if fewer remain. */
     ZD bool read(uint32_t c, uint32_t& v) {
@@ -154,10 +241,12 @@ struct BackBits {
     }
 };
 
+using LaneBits = BackBitsT<PerLane>;
+
 /* A forward little-endian bitstream (FSE table descriptions, RFC 8878
  * 4.1.1): bits from the low end of each byte up. */
 struct ForwardBits {
-    const uint8_t* p;
+    const ZG uint8_t* p;
     uint64_t size;
     uint64_t byte;
     uint32_t bit;
@@ -193,8 +282,8 @@ struct FseCell {
  * that is truncated, whose accuracy log exceeds log_max, that names a
  * symbol past max_symbol, or whose probabilities do not sum to the table
  * size (tests/zstd_fse_twin.cpp pins these rungs in the reference). */
-ZD bool read_fse_description(const uint8_t* src, uint64_t size, uint32_t max_symbol,
-                             uint32_t log_max, int16_t* counts, uint32_t& last_symbol,
+ZD bool read_fse_description(const ZG uint8_t* src, uint64_t size, uint32_t max_symbol,
+                             uint32_t log_max, ZL int16_t* counts, uint32_t& last_symbol,
                              uint32_t& log, uint64_t& used) {
     if (size == 0) {
         return false;
@@ -285,8 +374,8 @@ ZD bool read_fse_description(const uint8_t* src, uint64_t size, uint32_t max_sym
  * is scratch of last_symbol + 1 entries. Refuses counts that do not fill
  * the table exactly. */
 template <class W>
-ZD bool build_fse_table(const W& w, const int16_t* counts, uint32_t last_symbol, uint32_t log,
-                        FseCell* cells, uint16_t* next) {
+ZD bool build_fse_table(const W& w, const ZL int16_t* counts, uint32_t last_symbol, uint32_t log,
+                        ZL FseCell* cells, ZL uint16_t* next) {
     const uint32_t size = 1u << log;
     uint32_t claimed = 0;
     for (uint32_t s = 0; s <= last_symbol; s++) {
@@ -361,8 +450,8 @@ struct HufScratch {
  * above 11, a sum that is zero or leaves a non-power-of-two remainder, and
  * fewer than two symbols of weight one. */
 template <class W>
-ZD bool read_huffman_table(const W& w, const uint8_t* src, uint64_t size, HufScratch& s,
-                           uint16_t* table, uint32_t& table_log, uint64_t& used) {
+ZD bool read_huffman_table(const W& w, const ZG uint8_t* src, uint64_t size, ZL HufScratch& s,
+                           ZL uint16_t* table, uint32_t& table_log, uint64_t& used) {
     if (size == 0) {
         return false;
     }
@@ -393,7 +482,7 @@ ZD bool read_huffman_table(const W& w, const uint8_t* src, uint64_t size, HufScr
         if (!build_fse_table(w, s.counts, last, log, s.cells, s.next)) {
             return false;
         }
-        BackBits in;
+        BackBitsT<typename W::Same> in;
         if (!in.start(src + 1 + description, payload - description)) {
             return false;
         }
@@ -492,13 +581,34 @@ ZD bool read_huffman_table(const W& w, const uint8_t* src, uint64_t size, HufScr
 /* One Huffman stream of `count` symbols into out: the stream must start
  * with a marker, every code must be paid for by the bits left, and no bit
  * may be left over. */
-ZD bool decode_huffman_stream(const uint8_t* src, uint64_t size, const uint16_t* table,
-                              uint32_t table_log, uint8_t* out, uint64_t count) {
-    BackBits in;
+ZD bool decode_huffman_stream(const ZG uint8_t* src, uint64_t size, const ZL uint16_t* table,
+                              uint32_t table_log, ZG uint8_t* out, uint64_t count, ZL uint8_t* win,
+                              uint32_t win_cap) {
+    LaneBits in;
     if (!in.start(src, size)) {
         return false;
     }
-    for (uint64_t i = 0; i < count; i++) {
+    uint64_t i = 0;
+    /* While 8 or more stream bytes are still unloaded, a refill leaves at
+     * least 56 bits, which pays for four symbols of at most 11 bits: those
+     * decode with no tests, exactly as the loop below would. */
+    const uint32_t mask = (1u << table_log) - 1;
+    while (i + 4 <= count && in.pos >= 8) {
+        if (in.needs_stage()) {
+            in.stage_lane(win, win_cap);
+        }
+        in.refill();
+        for (uint32_t j = 0; j < 4; j++) {
+            const uint16_t cell = table[(uint32_t)(in.acc >> (in.n - table_log)) & mask];
+            in.n -= cell >> 8;
+            out[i + j] = (uint8_t)cell;
+        }
+        i += 4;
+    }
+    for (; i < count; i++) {
+        if (in.needs_stage()) {
+            in.stage_lane(win, win_cap); /* the next bytes, into this lane's window */
+        }
         const uint16_t cell = table[in.peek(table_log)];
         uint32_t v = 0;
         if (!in.read(cell >> 8, v)) {
@@ -512,9 +622,26 @@ ZD bool decode_huffman_stream(const uint8_t* src, uint64_t size, const uint16_t*
 /* ------------------------------------------------------------- sequences */
 
 struct SeqTable {
-    FseCell* cells;
+    ZL FseCell* cells;
     uint32_t log;     /* 0 for a run-length table of one cell */
     bool present;
+};
+
+/* Up to kBatch decoded and checked sequences of a block are executed
+ * together. One sequence's record: distance, literal length, match length,
+ * where its literals start in the block's literals, and where its output
+ * starts relative to the batch's first output byte. The Wave policy keeps
+ * the records (W::Records: put(lane, k, rec), get(k) for every lane,
+ * own(lane, k) for the lane that holds record k): on the device lane k
+ * holds record k in registers. */
+constexpr uint32_t kBatch = 64;
+/* A batch's bitstream is at most 64 x (31 + 16 + 16 extra bits + 9 + 9 + 8
+ * state bits) = 712 bytes, so a window of this size staged at a batch's
+ * start holds every byte its refills read. */
+constexpr uint32_t kWindowBytes = 1024;
+struct Rec {
+    uint64_t d;
+    uint32_t ll, ml, lit, at;
 };
 
 /* Frame state that survives block boundaries, and per-block scratch. */
@@ -527,16 +654,19 @@ struct FrameTables {
     int16_t counts[kMatchLenMax + 1];
     uint16_t next[kMatchLenMax + 1];
     uint32_t stream_bad[4]; /* per Huffman stream: 1 when it failed */
+    /* bitstream windows: a quarter per Huffman stream while literals are
+     * decoded, then the whole for the sequences */
+    uint8_t window[kWindowBytes];
 };
 
 /* Loads one field's table for a sequences section (RFC 8878
  * 3.1.1.3.2.1): 0 predefined, 1 run-length (one symbol byte), 2 an FSE
  * description, 3 repeat the previous table of this field. */
 template <class W>
-ZD bool load_seq_table(const W& w, uint32_t mode, const uint8_t* src, uint64_t size,
+ZD bool load_seq_table(const W& w, uint32_t mode, const ZG uint8_t* src, uint64_t size,
                        uint32_t max_symbol, uint32_t log_max, const int16_t* defaults,
-                       uint32_t default_last, uint32_t default_log, SeqTable& t, int16_t* counts,
-                       uint16_t* next, uint64_t& used) {
+                       uint32_t default_last, uint32_t default_log, SeqTable& t,
+                       ZL int16_t* counts, ZL uint16_t* next, uint64_t& used) {
     used = 0;
     if (mode == 3) {
         return t.present;
@@ -575,38 +705,34 @@ ZD bool load_seq_table(const W& w, uint32_t mode, const uint8_t* src, uint64_t s
 
 /* The frame's three repeat offsets (RFC 8878 3.1.1.5). */
 struct Repeats {
-    uint64_t r[3];
+    uint64_t r0, r1, r2; /* named, not an array: no indexed (memory) access */
 
     ZD void reset() {
-        r[0] = 1;
-        r[1] = 4;
-        r[2] = 8;
+        r0 = 1;
+        r1 = 4;
+        r2 = 8;
     }
     /* The match distance of an offset value, updating the history; 0 (an
-     * error) for a repeat that resolves to zero. */
+     * error) for a repeat that resolves to zero. Slot 3 stands for a new
+     * distance (an explicit offset, or the most recent one minus one). */
     ZD uint64_t resolve(uint64_t value, uint32_t literals) {
         uint64_t d = 0;
         uint32_t slot = 3;
         if (value > 3) {
             d = value - 3;
-        } else if (literals != 0) {
-            slot = (uint32_t)value - 1;
-            d = r[slot];
-        } else if (value < 3) {
-            slot = (uint32_t)value;
-            d = r[slot];
         } else {
-            d = r[0] != 0 ? r[0] - 1 : 0;
+            slot = literals != 0 ? (uint32_t)value - 1 : (uint32_t)value;
+            d = slot == 0 ? r0 : slot == 1 ? r1 : slot == 2 ? r2 : (r0 != 0 ? r0 - 1 : 0);
         }
         if (d == 0) {
             return 0;
         }
         if (slot != 0) {
             if (slot != 1) {
-                r[2] = r[1];
+                r2 = r1;
             }
-            r[1] = r[0];
-            r[0] = d;
+            r1 = r0;
+            r0 = d;
         }
         return d;
     }
@@ -622,7 +748,7 @@ struct FrameHeader {
 };
 
 /* RFC 8878 3.1.1.1. CUDEC_OK, CORRUPT_INPUT or UNSUPPORTED. */
-ZD int parse_frame_header(const uint8_t* src, uint64_t size, FrameHeader& h) {
+ZD int parse_frame_header(const ZG uint8_t* src, uint64_t size, FrameHeader& h) {
     if (size < 4) {
         return CUDEC_ERR_CORRUPT_INPUT;
     }
@@ -677,7 +803,8 @@ struct FrameEnd {
 /* Copies n literal bytes to out[0, n): from `lit` (raw or decoded), or the
  * byte `fill` when lit is null (a run-length section). */
 template <class W>
-ZD void put_literals(const W& w, uint8_t* out, const uint8_t* lit, uint8_t fill, uint64_t n) {
+ZD void put_literals(const W& w, ZG uint8_t* out, const ZG uint8_t* lit, uint8_t fill,
+                     uint64_t n) {
     if (lit != nullptr) {
         for (uint64_t i = w.lane; i < n; i += W::kLanes) {
             out[i] = lit[i];
@@ -693,8 +820,8 @@ ZD void put_literals(const W& w, uint8_t* out, const uint8_t* lit, uint8_t fill,
 /* out[at + i] = out[at - d + (i mod d)] for i < n: every source byte lies
  * before `at`, so lanes need no order among themselves. */
 template <class W>
-ZD void put_match(const W& w, uint8_t* out, uint64_t at, uint64_t d, uint64_t n) {
-    const uint8_t* from = out + at - d;
+ZD void put_match(const W& w, ZG uint8_t* out, uint64_t at, uint64_t d, uint64_t n) {
+    const ZG uint8_t* from = out + at - d;
     if (d <= 0xFFFFFFFFull && n <= 0xFFFFFFFFull) {
         const uint32_t d32 = (uint32_t)d;
         for (uint32_t i = w.lane; i < (uint32_t)n; i += W::kLanes) {
@@ -708,13 +835,129 @@ ZD void put_match(const W& w, uint8_t* out, uint64_t at, uint64_t d, uint64_t n)
     w.sync();
 }
 
+/* The wave copies stream bytes [pos - cap, pos) (from 0 when pos < cap)
+ * into buf, and the reader reads from there: a sequence batch's bits in one
+ * round trip. */
+template <class W>
+ZD void stage_wave(const W& w, BackBitsT<typename W::Same>& in, ZL uint8_t* buf, uint32_t cap) {
+    const uint64_t lo = in.pos > cap ? in.pos - cap : 0;
+    const uint32_t len = (uint32_t)(in.pos - lo);
+    for (uint32_t i = w.lane * 16; i < len; i += W::kLanes * 16) {
+        if (i + 16 <= len) {
+            uint8_t v[16];
+            __builtin_memcpy(v, (const void*)(in.p + lo + i), 16);
+            for (uint32_t j = 0; j < 16; j++) {
+                buf[i + j] = v[j];
+            }
+        } else {
+            for (uint32_t j = i; j < len; j++) {
+                buf[j] = in.p[lo + j];
+            }
+        }
+    }
+    w.sync();
+    in.win = buf;
+    in.win_lo = lo;
+    in.win_hi = in.pos;
+}
+
+/* An FSE cell as one 32-bit load, made wave-uniform. */
+template <class W>
+ZD FseCell load_cell(const W& w, const ZL FseCell* cells, uint32_t state) {
+    const FseCell raw = cells[state];
+    uint32_t v;
+    __builtin_memcpy(&v, &raw, 4);
+    v = w.uniform(v);
+    FseCell c;
+    __builtin_memcpy(&c, &v, 4);
+    return c;
+}
+
+/* n bytes from src to dst on one lane: 16-byte steps, then single bytes
+ * (never past dst + n: the bytes after belong to other lanes). */
+ZD void lane_copy(ZG uint8_t* dst, const ZG uint8_t* src, uint32_t n) {
+    uint32_t i = 0;
+    for (; i + 16 <= n; i += 16) {
+        uint8_t v[16];
+        __builtin_memcpy(v, (const void*)(src + i), 16);
+        __builtin_memcpy((void*)(dst + i), v, 16);
+    }
+    for (; i < n; i++) {
+        dst[i] = src[i];
+    }
+}
+
+/* Literals and matches a single lane takes alone. */
+constexpr uint32_t kLaneBytes = 64;
+
+/* Executes a batch of cnt sequences whose output starts at out[bo]:
+ *   1. one lane a sequence: literals of up to kLaneBytes, and matches of up
+ *      to kLaneBytes whose source period lies wholly before the batch (so
+ *      already written): neither reads anything the batch writes;
+ *   2. the wave, sequence by sequence: longer literals, and longer matches
+ *      whose period lies before the batch;
+ *   3. the wave, in sequence order: every other match, whose source may be
+ *      the output of an earlier sequence of the batch (all of which steps 1
+ *      and 2 wrote, or step 3 wrote before it).
+ * A match's bytes depend only on bytes before its own start, so this order
+ * writes what sequence-by-sequence execution writes. */
+template <class W>
+ZD void execute_batch(const W& w, ZG uint8_t* out, uint64_t bo, const ZG uint8_t* lit,
+                      uint8_t fill, const typename W::Records& recs, uint32_t cnt) {
+    ZG uint8_t* base = out + bo;
+    for (uint32_t k = w.lane; k < cnt; k += W::kLanes) {
+        const Rec r = recs.own(w.lane, k);
+        if (r.ll <= kLaneBytes) {
+            if (lit != nullptr) {
+                lane_copy(base + r.at, lit + r.lit, r.ll);
+            } else {
+                for (uint32_t i = 0; i < r.ll; i++) {
+                    base[r.at + i] = fill;
+                }
+            }
+        }
+        const uint64_t m = (uint64_t)r.at + r.ll;
+        const uint64_t span = r.d < r.ml ? r.d : r.ml;
+        if (r.ml <= kLaneBytes && r.d >= m + span) {
+            const ZG uint8_t* from = base + m - r.d; /* before the batch: d <= bo + m */
+            if (r.d >= r.ml) {
+                lane_copy(base + m, from, r.ml);
+            } else {
+                for (uint32_t i = 0; i < r.ml; i++) {
+                    base[m + i] = from[i % (uint32_t)r.d];
+                }
+            }
+        }
+    }
+    w.sync();
+    for (uint32_t k = 0; k < cnt; k++) {
+        const Rec r = recs.get(k);
+        if (r.ll > kLaneBytes) {
+            put_literals(w, base + r.at, lit != nullptr ? lit + r.lit : nullptr, fill, r.ll);
+        }
+        const uint64_t m = (uint64_t)r.at + r.ll;
+        const uint64_t span = r.d < r.ml ? r.d : r.ml;
+        if (r.ml > kLaneBytes && r.d >= m + span) {
+            put_match(w, out, bo + m, r.d, r.ml);
+        }
+    }
+    for (uint32_t k = 0; k < cnt; k++) {
+        const Rec r = recs.get(k);
+        const uint64_t m = (uint64_t)r.at + r.ll;
+        const uint64_t span = r.d < r.ml ? r.d : r.ml;
+        if (r.d < m + span) {
+            put_match(w, out, bo + m, r.d, r.ml);
+        }
+    }
+}
+
 /* Decodes one frame of `size` bytes into out[0, cap). `lits` is scratch of
  * kBlockCeiling bytes for Huffman-coded literals. Returns the status; on
  * CUDEC_OK `end` says how much was written and which checksum the output
  * must have. */
 template <class W>
-ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out, uint64_t cap,
-                    FrameTables& t, uint8_t* lits, FrameEnd& end) {
+ZD int decode_frame(const W& w, const ZG uint8_t* src, uint64_t size, ZG uint8_t* out,
+                    uint64_t cap, ZL FrameTables& t, ZG uint8_t* lits, FrameEnd& end) {
     FrameHeader h;
     int st = parse_frame_header(src, size, h);
     if (st != CUDEC_OK) {
@@ -746,7 +989,7 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
         if (size - pos - 3 < body_size) {
             return CUDEC_ERR_CORRUPT_INPUT;
         }
-        const uint8_t* body = src + pos + 3;
+        const ZG uint8_t* body = src + pos + 3;
         if (type == 0 || type == 1) {
             if (bsize > cap - o) {
                 return CUDEC_ERR_OUTPUT_TOO_SMALL;
@@ -784,7 +1027,7 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
                 return CUDEC_ERR_CORRUPT_INPUT;
             }
             const uint64_t after = body_size - lh;
-            const uint8_t* lit = nullptr; /* null: run-length literals */
+            const ZG uint8_t* lit = nullptr; /* null: run-length literals */
             uint8_t fill = 0;
             uint64_t lused = 0;
             if (ltype == 0) {
@@ -803,7 +1046,7 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
                 if (csize > after) {
                     return CUDEC_ERR_CORRUPT_INPUT;
                 }
-                const uint8_t* payload = body + lh;
+                const ZG uint8_t* payload = body + lh;
                 uint64_t psize = csize;
                 if (ltype == 2) {
                     uint32_t log = 0;
@@ -829,7 +1072,8 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
                 if (streams == 1) {
                     for (uint32_t s = w.lane; s < 1; s += W::kLanes) {
                         t.stream_bad[0] = decode_huffman_stream(payload, psize, t.huffman, huf_log,
-                                                                lits, regen) ? 0u : 1u;
+                                                                lits, regen, t.window,
+                                                                kWindowBytes) ? 0u : 1u;
                     }
                 } else {
                     if (psize < 10 || regen < 6) {
@@ -846,8 +1090,10 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
                         const uint64_t from = 6 + (s > 0 ? s0 : 0) + (s > 1 ? s1 : 0) + (s > 2 ? s2 : 0);
                         const uint64_t len = s == 0 ? s0 : s == 1 ? s1 : s == 2 ? s2 : bytes - s0 - s1 - s2;
                         const uint64_t n = s < 3 ? seg : regen - 3 * seg;
-                        t.stream_bad[s] = decode_huffman_stream(payload + from, len, t.huffman,
-                                                                huf_log, lits + s * seg, n) ? 0u : 1u;
+                        t.stream_bad[s] = decode_huffman_stream(
+                                              payload + from, len, t.huffman, huf_log,
+                                              lits + s * seg, n, t.window + s * (kWindowBytes / 4),
+                                              kWindowBytes / 4) ? 0u : 1u;
                     }
                 }
                 w.sync();
@@ -859,7 +1105,7 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
             }
 
             /* sequences section header (RFC 8878 3.1.1.3.2.1) */
-            const uint8_t* sec = body + lused;
+            const ZG uint8_t* sec = body + lused;
             uint64_t left = body_size - lused;
             if (left == 0) {
                 return CUDEC_ERR_CORRUPT_INPUT;
@@ -916,10 +1162,11 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
                 left -= u;
 
                 /* the sequences (RFC 8878 3.1.1.3.2.2 and 4.1) */
-                BackBits in;
+                BackBitsT<typename W::Same> in;
                 if (left == 0 || !in.start(sec, left)) {
                     return CUDEC_ERR_CORRUPT_INPUT;
                 }
+                stage_wave(w, in, t.window, kWindowBytes);
                 uint32_t sl = 0, so = 0, sm = 0;
                 if (!in.read(ll.log, sl) || !in.read(of.log, so) || !in.read(ml.log, sm)) {
                     return CUDEC_ERR_CORRUPT_INPUT;
@@ -927,11 +1174,18 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
                 const uint32_t nl = 1u << ll.log, no = 1u << of.log, nm = 1u << ml.log;
                 int pending = CUDEC_OK; /* the first execution error, reported
                                            once the whole block decodes */
+                typename W::Records recs;
+                uint32_t cnt = 0;     /* sequences in the batch */
+                uint64_t bo = o;      /* the batch's first output byte */
                 for (uint32_t k = 0; k < count; k++) {
+                    if (k != 0 && (k & (kBatch - 1)) == 0) {
+                        stage_wave(w, in, t.window, kWindowBytes);
+                    }
                     if (sl >= nl || so >= no || sm >= nm) {
                         return CUDEC_ERR_CORRUPT_INPUT;
                     }
-                    const FseCell cl = ll.cells[sl], co = of.cells[so], cm = ml.cells[sm];
+                    const FseCell cl = load_cell(w, ll.cells, sl), co = load_cell(w, of.cells, so),
+                                  cm = load_cell(w, ml.cells, sm);
                     if (cl.symbol > kLitLenMax || co.symbol > kOffsetMax || cm.symbol > kMatchLenMax) {
                         return CUDEC_ERR_CORRUPT_INPUT;
                     }
@@ -976,16 +1230,28 @@ ZD int decode_frame(const W& w, const uint8_t* src, uint64_t size, uint8_t* out,
                         pending = CUDEC_ERR_OUTPUT_TOO_SMALL;
                         continue;
                     }
-                    put_literals(w, out + o, lit != nullptr ? lit + lp : nullptr, fill, llen);
-                    put_match(w, out, m, d, mlen);
+                    recs.put(w.lane, cnt, Rec{d, llen, mlen, (uint32_t)lp, (uint32_t)(o - bo)});
+                    cnt++;
                     lp += llen;
                     o = m + mlen;
+                    if (cnt == kBatch) {
+                        w.sync();
+                        execute_batch(w, out, bo, lit, fill, recs, cnt);
+                        w.sync();
+                        cnt = 0;
+                        bo = o;
+                    }
                 }
                 if (!in.at_end()) {
                     return CUDEC_ERR_CORRUPT_INPUT;
                 }
                 if (pending != CUDEC_OK) {
                     return pending;
+                }
+                if (cnt != 0) {
+                    w.sync();
+                    execute_batch(w, out, bo, lit, fill, recs, cnt);
+                    w.sync();
                 }
             }
             /* the literals after the last sequence */

@@ -19,7 +19,9 @@
  *     over, a match reaching past the window or before the output, output
  *     other than the declared content size, a failed XXH64 content
  *     checksum, or bytes after the frame.
- * On success bytes_written is the declared content size. */
+ * On success bytes_written is the declared content size. A chunk_count
+ * above UINT32_MAX is refused with CUDEC_ERR_INVALID_ARGUMENT, like the
+ * other argument errors, before any device call. */
 #ifndef CUDEC_ZSTD_H
 #define CUDEC_ZSTD_H
 

@@ -14,6 +14,15 @@ struct OneLane {
     static constexpr uint32_t kLanes = 1;
     uint32_t lane = 0;
     void sync() const {}
+    uint32_t uniform(uint32_t v) const { return v; }
+    using Same = cudec_amd::zstd::PerLane; /* one lane: its values are the wave's */
+    /* a batch's sequence records, all held by the one lane */
+    struct Records {
+        cudec_amd::zstd::Rec r[cudec_amd::zstd::kBatch];
+        void put(uint32_t, uint32_t k, const cudec_amd::zstd::Rec& v) { r[k] = v; }
+        cudec_amd::zstd::Rec get(uint32_t k) const { return r[k]; }
+        cudec_amd::zstd::Rec own(uint32_t, uint32_t k) const { return r[k]; }
+    };
 };
 
 }  // namespace
