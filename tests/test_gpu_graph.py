@@ -1,4 +1,4 @@
-"""A tuned batch launch captured into a HIP graph and replayed
+"""A tuned batch launch (LZ4, Snappy, Zstandard) captured into a HIP graph and replayed
 (cudec_amd/graph.py over the unchanged batch entries). Every replay must
 give the eager launch's statuses, bytes_written and output bytes, with the
 program's own eager work and synchronisations between replays; malformed
@@ -31,7 +31,8 @@ def mixed_streams(codec):
 
 
 @pytest.mark.parametrize("fmt,codec", [("lz4", workload.CODEC_LZ4),
-                                       ("snappy", workload.CODEC_SNAPPY)])
+                                       ("snappy", workload.CODEC_SNAPPY),
+                                       ("zstd", workload.CODEC_ZSTD)])
 def test_graph_replays_equal_the_eager_launch(fmt, codec):
     b, comp, caps = mixed_streams(codec)
     print(f"[graph] {fmt}: {b.n} chunks, {sum(caps)} output bytes", flush=True)

@@ -98,3 +98,25 @@ def test_many_frames():
     for i, d in enumerate(origs):
         assert int(res["bytes_written"][i]) == len(d)
         assert db.slot(arena, i)[:len(d)].tobytes() == d, i
+
+
+def test_two_streams_at_once():
+    """Two batches decoding at the same time on two streams, each with its
+    own scratch: both equal libzstd's output."""
+    import torch
+    from cudec_amd.device import DeviceBatch
+    from cudec_amd import workload
+    batches = [workload.uneven(300, seed=s, codec=workload.CODEC_ZSTD, octaves=8)
+               for s in (0x51, 0x52)]
+    dbs = [DeviceBatch(b.comp, b.comp_off, b.comp_size, b.sizes, out_off=b.orig_off)
+           for b in batches]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    for db, s in zip(dbs, streams):
+        assert db.run("zstd", "tuned", stream=s) == 0
+    torch.cuda.synchronize()
+    for b, db in zip(batches, dbs):
+        res = db.results()
+        assert np.all(res["status"] == libs.OK)
+        assert np.array_equal(res["bytes_written"], b.sizes)
+        assert np.array_equal(db.output_arena()[:b.orig.size], b.orig)
