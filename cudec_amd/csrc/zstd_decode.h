@@ -109,19 +109,17 @@ ZD uint64_t read_le(const ZG uint8_t* p, uint32_t n) {
 
 /* ------------------------------------------------------------ bitstreams */
 
-/* A backward bitstream (RFC 8878 4.1): read from the last byte towards the
- * first, starting below the highest set bit of the last byte. `acc` holds
- * the next `n` bits in its low bits, the next bit highest; bytes [0, pos)
- * are not loaded yet. */
-/* How a reader's loaded words are held: PerLane keeps each lane's own
- * (a lane decoding its own stream); the Wave policy's Same makes them one
- * wave-uniform value (every lane decoding the same stream), so the
- * reader's state lives in scalar registers and its tests are scalar
- * branches. */
+/* How a reader's loaded words are held. PerLane: as loaded (a lane decoding
+ * its own stream). A reader every lane runs on the same stream takes the
+ * Wave policy's Same instead, which may make them one wave-uniform value. */
 struct PerLane {
     static ZD uint64_t fix64(uint64_t v) { return v; }
 };
 
+/* A backward bitstream (RFC 8878 4.1): read from the last byte towards the
+ * first, starting below the highest set bit of the last byte. `acc` holds
+ * the next `n` bits in its low bits, the next bit highest; bytes [0, pos)
+ * are not loaded yet. */
 template <class U>
 struct BackBitsT {
     const ZG uint8_t* p;
@@ -153,7 +151,7 @@ struct BackBitsT {
     }
     ZD uint64_t remaining() const { return n + 8 * pos; }
     ZD bool at_end() const { return n == 0 && pos == 0; }
-    /* Tops the accumulator up to at least 57 bits (or to the stream's
+    /* Tops the accumulator up to at least 56 bits (or to the stream's
      * start): one 8-byte load takes the next k <= 7 bytes, so a refill costs
      * one memory round trip, not one per byte. */
     ZD void refill() {
