@@ -388,27 +388,26 @@ __device__ __forceinline__ bool period_in_history(uint64_t m, uint32_t off, uint
     return y + need <= hist_lo && y + ((need + 15) & ~15u) <= cap;
 }
 
-/* Up to four 16-byte global loads of a history period, issued together. */
+/* Four 16-byte global loads of a history period of need (1..64) bytes at
+ * `at`, issued together: block k starts 16 k bytes in, or at the period's
+ * last block where the period is shorter, so every load stays inside the
+ * period's 16-byte blocks. Straight-line on purpose: a load under a
+ * condition of its own leaves the compiler copying its result where the
+ * condition ends, and the copy waits for the load right there. */
 struct Period {
     uint4 h0, h1, h2, h3;
 };
-__device__ __forceinline__ Period load_period(gcu8* dst, uint64_t y, uint32_t need) {
+__device__ __forceinline__ Period load_period(gcu8* at, uint32_t need) {
+    const uint32_t last = (need - 1) & ~15u;
     Period p;
-    p.h1 = make_uint4(0, 0, 0, 0);
-    p.h2 = make_uint4(0, 0, 0, 0);
-    p.h3 = make_uint4(0, 0, 0, 0);
-    p.h0 = gload16(dst + y);
-    if (need > 16) {
-        p.h1 = gload16(dst + y + 16);
-    }
-    if (need > 32) {
-        p.h2 = gload16(dst + y + 32);
-    }
-    if (need > 48) {
-        p.h3 = gload16(dst + y + 48);
-    }
+    p.h0 = gload16(at);
+    p.h1 = gload16(at + (last < 16u ? last : 16u));
+    p.h2 = gload16(at + (last < 32u ? last : 32u));
+    p.h3 = gload16(at + last);
     return p;
 }
+/* What lanes with no history period load instead (never written). */
+__device__ __align__(16) uint8_t no_period[16];
 
 /* One lane copies a match of len (<= kLaneMatch) bytes at output position
  * m with offset off into the ring; every source byte precedes m or is
@@ -653,16 +652,6 @@ __device__ __forceinline__ void clear_bit(uint64_t& mask, uint32_t j) {
     asm("s_bitset0_b64 %0, %1" : "+s"(mask) : "s"(j));
 }
 
-/* lane mod d for lane < 64 and d >= 1: the float quotient lane * (1 / d)
- * never exceeds the true one (the reciprocal is within one ulp) and falls
- * short of it by at most one, at exact multiples, so one correction
- * suffices. For d > 64 the quotient is 0, so a 24-bit product is exact. */
-__device__ __forceinline__ uint32_t lane_mod(uint32_t lane, float lane_f, uint32_t d) {
-    const float inv = __builtin_amdgcn_rcpf((float)d);
-    const uint32_t r = lane - __umul24(d, (uint32_t)(lane_f * inv));
-    return r >= d ? r - d : r;
-}
-
 /* 16 ring bytes from index r, wrapping at the ring's end. */
 template <class W>
 __device__ __forceinline__ uint4 ring_read16(const W& w, uint32_t r) {
@@ -694,9 +683,10 @@ __device__ __forceinline__ uint4 take_bytes(const uint4& a, const uint4& b, uint
                       (a.z & m[2]) | (b.z & ~m[2]), (a.w & m[3]) | (b.w & ~m[3]));
 }
 
-/* x mod d for x < 4096 and d >= 1, by the float reciprocal as in lane_mod
- * (the quotient's error stays below one part in 2^11, so it falls short of
- * the true one by at most one, and only at exact multiples). */
+/* x mod d for x < 4096 and d >= 1, by the float reciprocal: the quotient
+ * x * (1 / d) never exceeds the true one (the reciprocal is within one
+ * ulp) and its error stays below one part in 2^11, so it falls short by at
+ * most one, and only at exact multiples; one correction suffices. */
 __device__ __forceinline__ uint32_t small_mod(uint32_t x, uint32_t d) {
     const float inv = __builtin_amdgcn_rcpf((float)d);
     const uint32_t r = x - __umul24(d, (uint32_t)((float)x * inv));
@@ -830,10 +820,11 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
     const bool lane_match = full && !wide_lit && match_len <= kLaneMatch && independent;
     const bool from_history =
         lane_match && period_in_history(m_at, offset, match_len, hist_lo, cap);
-    Period hp;
-    if (from_history) {
-        hp = load_period(dst, y, (uint32_t)need);
-    }
+    /* every lane loads, with no branch around the loads (whose results
+     * would then be copied, waiting for them here); a lane without a
+     * history period reads the zero block no_period */
+    const Period hp = load_period(from_history ? dst + y : (gcu8*)no_period,
+                                  from_history ? (uint32_t)need : 1u);
     /* phase 1: the lanes' literals */
     if (active && !wide_lit) {
         put_literal(w, cw, lit, lit_len, da, lit_at);
@@ -848,17 +839,22 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
      * short match (<= 64 bytes) whose period is in the ring, is described by
      * its lane up front in two words, so the loop pays two v_readlane and a
      * byte a lane for it: fast_a = ring index of the match (13 bits), of
-     * its period (13 bits), length - 1 (6 bits); fast_b = min(offset, 64).
-     * Byte i copies period byte i mod offset (i < 64; the float quotient
-     * is exact after one correction), which precedes the match, so the
-     * lanes do not depend on each other. */
+     * its period (13 bits), length - 1 (6 bits); fast_b = o = min(offset,
+     * 64) in bits 0-7 and ceil(4096 / o) above them. Byte i copies period
+     * byte i mod offset, which precedes the match, so the lanes do not
+     * depend on each other. For i <= 64 the quotient floor(i / o) is
+     * (i * ceil(4096 / o)) >> 12: with ceil(4096 / o) = (4096 + e) / o,
+     * 0 <= e < o, the scaled product is i / o + i * e / (4096 o), and
+     * i * e < 64 * 64 keeps the excess below the 1 / o that separates i / o
+     * from the next integer. Two 24-bit multiplies and a shift replace a
+     * reciprocal and its correction. */
     const bool rest_fast = full && !lane_match && !wide_lit && match_len != 0 &&
                            match_len <= kWave && y >= hist_lo;
     static_assert(W::kSize <= 8192, "ring indices are packed in 13 bits");
     const uint32_t fast_a = ring_index<W::kSize>(da, m_at) | ring_index<W::kSize>(da, y) << 13 |
                             (match_len - 1) << 26;
-    const uint32_t fast_b = offset < kWave ? offset : kWave;
-    const float lane_f = (float)lane;
+    const uint32_t fast_o = offset - 1 < kWave ? offset : kWave;
+    const uint32_t fast_b = fast_o | ((4096u + fast_o - 1) / fast_o) << 8;
     /* in stream order: runs of short ring matches (one straight-line step
      * each, in a loop with no other branch), each run ended by the next
      * other member */
@@ -872,13 +868,14 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
             const uint32_t j = __builtin_ctzll(run);
             clear_bit(run, j);
             const uint32_t ja = __builtin_amdgcn_readlane(fast_a, j);
-            const uint32_t jo = __builtin_amdgcn_readlane(fast_b, j);
+            const uint32_t jb = __builtin_amdgcn_readlane(fast_b, j);
+            const uint32_t jo = jb & 255;
             const uint32_t rm = ja & 8191;
             const uint32_t rs = (ja >> 13) & 8191;
             const uint32_t jm = (ja >> 26) + 1;
             /* lane mod jo (lanes below jo get a zero quotient); lanes past
              * the match write their own sink byte, so no lane is masked */
-            const uint32_t r = lane_mod(lane, lane_f, jo);
+            const uint32_t r = lane - __umul24(__umul24(lane, jb >> 8) >> 12, jo);
             uint8_t* at = lane < jm ? &w.ring[(rm + lane) & (W::kSize - 1)] : &w.sink[lane];
             *at = w.ring[(rs + r) & (W::kSize - 1)];
             wave_order();
@@ -906,9 +903,14 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
             const uint64_t jm0 = jd0 + jl;
             const uint64_t jfrom = jm0 - joff;
             /* byte i copies period byte i mod joff, kept per lane as i
-             * steps by 64 */
-            const uint32_t step = kWave % joff;
-            uint32_t r = lane_mod(lane, lane_f, joff);
+             * steps by 64; lane mod joff and 64 mod joff as in the fast
+             * loop (for joff > 64 they are lane and 64) */
+            const uint32_t jb = __builtin_amdgcn_readlane(fast_b, j);
+            const uint32_t jo = jb & 255;
+            const uint32_t jq = jb >> 8;
+            uint32_t r = lane - __umul24(__umul24(lane, jq) >> 12, jo);
+            const uint32_t step =
+                joff > kWave ? kWave : kWave - __umul24(__umul24(kWave, jq) >> 12, jo);
             for (uint32_t i = lane; i < jm; i += kWave) {
                 w.ring[ring_index<W::kSize>(da, jm0 + i)] =
                     history_byte(w, dst, jfrom + r, hist_lo);
