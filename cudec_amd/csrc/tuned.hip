@@ -454,15 +454,34 @@ __device__ __forceinline__ void lane_match_copy(W& w, gcu8* dst, uintptr_t da,
                 __builtin_memcpy(&h3, &w.ring[ry + 48], 16);
             }
         }
-        ring_put(w, rd, h0, need < 16 ? need : 16);
-        if (need > 16) {
-            ring_put(w, (rd + 16) & (W::kSize - 1), h1, need - 16 < 16 ? need - 16 : 16);
-        }
-        if (need > 32) {
-            ring_put(w, (rd + 32) & (W::kSize - 1), h2, need - 32 < 16 ? need - 32 : 16);
-        }
-        if (need > 48) {
-            ring_put(w, (rd + 48) & (W::kSize - 1), h3, need - 48);
+        if (rd + need <= W::kSize) {
+            /* whole blocks, then one exact tail */
+            const uint32_t full = need >> 4;
+            if (full > 0) {
+                __builtin_memcpy(&w.ring[rd], &h0, 16);
+            }
+            if (full > 1) {
+                __builtin_memcpy(&w.ring[rd + 16], &h1, 16);
+            }
+            if (full > 2) {
+                __builtin_memcpy(&w.ring[rd + 32], &h2, 16);
+            }
+            if (full > 3) {
+                __builtin_memcpy(&w.ring[rd + 48], &h3, 16);
+            }
+            const uint4 t = full == 0 ? h0 : full == 1 ? h1 : full == 2 ? h2 : h3;
+            ring_put(w, rd + 16 * full, t, need & 15);
+        } else {
+            ring_put(w, rd, h0, need < 16 ? need : 16);
+            if (need > 16) {
+                ring_put(w, (rd + 16) & (W::kSize - 1), h1, need - 16 < 16 ? need - 16 : 16);
+            }
+            if (need > 32) {
+                ring_put(w, (rd + 32) & (W::kSize - 1), h2, need - 32 < 16 ? need - 32 : 16);
+            }
+            if (need > 48) {
+                ring_put(w, (rd + 48) & (W::kSize - 1), h3, need - 48);
+            }
         }
     } else {
         for (uint32_t i = 0; i < need; i++) {
@@ -937,8 +956,9 @@ __device__ __forceinline__ void execute_group(W& w, const uint8_t* cw, const Gro
  * is not a member), past the window, or after 64 members; it returns the
  * position after the last member and the member count. */
 template <uint32_t kC, uint32_t kMinStep, bool kStraight>
-__device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint32_t start,
-                                               uint32_t& count_out, uint32_t& member) {
+__device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC],
+                                                       uint32_t start, uint32_t& count_out,
+                                                       uint32_t& member) {
     static_assert(kStop >= kC * kWave + kWave, "a stop link must leave every region");
     static_assert(kMinStep >= 2, "links go forward");
     /* most members one region can hold, and the count at or below which
@@ -1046,6 +1066,152 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
     count_out = uniform32(count);
     return uniform32(pos);
 }
+
+/* One region k of the walk (positions [64 k, 64 k + 64), links in lk<k>):
+ * per member, read the link of pos out of its lane, record pos as member
+ * m0, step; two members a turn, the current position alternating between
+ * pos and nx so that no step moves it. Links only go forward by kMinStep
+ * bytes or more, so a region holds at most kRegionMax members and
+ * "pos < 64 k + 64" is the whole region check; entered with at most
+ * kFastMax members the walk cannot pass 64 inside the region and takes the
+ * fast loop (the position itself selects the lane: a wave64 v_readlane
+ * uses bits [5:0] of the select), else the checked loop, which tests the
+ * count as well. Four instructions separate a v_readlane from the one that
+ * uses its result as the select (the 4 wait states that use needs), and
+ * three separate an m0 update from the v_writelane that reads it. On
+ * leaving the region (pos = the next position): a stop link or 64 members
+ * end the walk, else the next region's check follows. */
+#define WALK_REGION(k, B, L, N)                                                             \
+    "r" #k "_%=:\n\t"                                                                       \
+    "s_cmp_lt_u32 %[pos], " #L "\n\t"                                                       \
+    "s_cbranch_scc0 r" #N "_%=\n\t"                                                         \
+    "s_sub_u32 %[rel], %[pos], " #B "\n\t"                                                  \
+    "s_cmp_gt_u32 m0, %[fmax]\n\t"                                                          \
+    "s_cbranch_scc1 c" #k "_%=\n"                                                           \
+    "f" #k "_%=:\n\t"                                                                       \
+    "v_readlane_b32 %[nx], %[lk" #k "], %[pos]\n\t"                                         \
+    "v_writelane_b32 %[mem], %[pos], m0\n\t"                                                \
+    "s_add_u32 m0, m0, 1\n\t"                                                               \
+    "s_cmp_lt_u32 %[nx], " #L "\n\t"                                                        \
+    "s_cbranch_scc0 o" #k "_%=\n\t"                                                         \
+    "v_readlane_b32 %[pos], %[lk" #k "], %[nx]\n\t"                                         \
+    "v_writelane_b32 %[mem], %[nx], m0\n\t"                                                 \
+    "s_add_u32 m0, m0, 1\n\t"                                                               \
+    "s_cmp_lt_u32 %[pos], " #L "\n\t"                                                       \
+    "s_cbranch_scc1 f" #k "_%=\n\t"                                                         \
+    "s_mov_b32 %[prev], %[nx]\n\t"                                                          \
+    "s_branch x" #k "_%=\n"                                                                 \
+    "c" #k "_%=:\n\t"                                                                       \
+    "v_readlane_b32 %[nx], %[lk" #k "], %[rel]\n\t"                                         \
+    "v_writelane_b32 %[mem], %[pos], m0\n\t"                                                \
+    "s_sub_u32 %[rel], %[nx], " #B "\n\t"                                                   \
+    "s_add_u32 m0, m0, 1\n\t"                                                               \
+    "s_or_b32 %[t], %[rel], m0\n\t"                                                         \
+    "s_cmp_lt_u32 %[t], 64\n\t"                                                             \
+    "s_cbranch_scc0 o" #k "_%=\n\t"                                                         \
+    "v_readlane_b32 %[pos], %[lk" #k "], %[rel]\n\t"                                        \
+    "v_writelane_b32 %[mem], %[nx], m0\n\t"                                                 \
+    "s_sub_u32 %[rel], %[pos], " #B "\n\t"                                                  \
+    "s_add_u32 m0, m0, 1\n\t"                                                               \
+    "s_or_b32 %[t], %[rel], m0\n\t"                                                         \
+    "s_cmp_lt_u32 %[t], 64\n\t"                                                             \
+    "s_cbranch_scc1 c" #k "_%=\n\t"                                                         \
+    "s_mov_b32 %[prev], %[nx]\n\t"                                                          \
+    "s_branch x" #k "_%=\n"                                                                 \
+    "o" #k "_%=:\n\t"                                                                       \
+    "s_mov_b32 %[prev], %[pos]\n\t"                                                         \
+    "s_mov_b32 %[pos], %[nx]\n"                                                             \
+    "x" #k "_%=:\n\t"                                                                       \
+    "s_cmpk_eq_u32 %[pos], 0xffff\n\t"                                                      \
+    "s_cbranch_scc1 stop_%=\n\t"                                                            \
+    "s_cmp_eq_u32 m0, 64\n\t"                                                               \
+    "s_cbranch_scc1 full_%=\n\t"                                                            \
+    "s_branch r" #N "_%=\n"
+
+/* The walk's entry and exits around the regions: the count lives in m0
+ * during the walk (saved and restored around it): v_writelane takes its
+ * value and lane select from scalar registers, and only m0 as the lane
+ * select keeps it within one constant-bus read. The leading s_nop covers a
+ * lane select the compiler may have just written from a vector instruction,
+ * and m0 written just before its first use. A stop link: prev is no
+ * member, and the walk ends there; past the last region: the window ends. */
+#define WALK_BEGIN                                                                          \
+    "s_mov_b32 %[m0s], m0\n\t"                                                              \
+    "s_mov_b32 m0, %[cnt]\n\t"                                                              \
+    "s_nop 4\n"
+#define WALK_END(N)                                                                         \
+    "r" #N "_%=:\n\t"                                                                       \
+    "s_mov_b32 %[fin], 0\n\t"                                                               \
+    "s_branch end_%=\n"                                                                     \
+    "stop_%=:\n\t"                                                                          \
+    "s_mov_b32 %[pos], %[prev]\n\t"                                                         \
+    "s_mov_b32 %[fin], 1\n\t"                                                               \
+    "s_sub_u32 m0, m0, 1\n\t"                                                               \
+    "s_branch end_%=\n"                                                                     \
+    "full_%=:\n\t"                                                                          \
+    "s_mov_b32 %[fin], 1\n"                                                                 \
+    "end_%=:\n\t"                                                                           \
+    "s_mov_b32 %[cnt], m0\n\t"                                                              \
+    "s_mov_b32 m0, %[m0s]"
+#define WALK_OPERANDS                                                                       \
+    : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+s"(count), [rel] "=&s"(rel),             \
+      [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [m0s] "=&s"(m0_saved),              \
+      [fin] "=&s"(fin)
+
+/* The walk of walk_chain_regions, with the same results: for windows of up
+ * to eight regions as one asm block (WALK_REGION per region), else region
+ * by region. */
+template <uint32_t kC, uint32_t kMinStep, bool kStraight>
+__device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint32_t start,
+                                               uint32_t& count_out, uint32_t& member) {
+    static_assert(kStop >= kC * kWave + kWave, "a stop link must leave every region");
+    static_assert(kMinStep >= 2, "links go forward");
+    (void)kStraight;
+    /* most members one region can hold, and the count at or below which
+     * the walk takes the unchecked loop there */
+    constexpr uint32_t kRegionMax = (kWave - 1) / kMinStep + 1;
+    constexpr uint32_t kFastMax = kWave - kRegionMax;
+    if constexpr (kC > 8) {
+        /* sixteen regions in one block keep all sixteen link registers live
+         * at once and spill (5% slower on LZ4 than a block per region) */
+        return walk_chain_regions<kC, kMinStep, kStraight>(link, start, count_out, member);
+    } else {
+        uint32_t pos = start, count = 0, fin = 0, rel, prev = 0, nx, t, m0_saved;
+        /* one asm block for the whole walk: no compiler code between regions
+         * (Snappy's seven regions: 2% faster than a block per region) */
+        if constexpr (kC == 8) {
+            asm volatile(WALK_BEGIN WALK_REGION(0, 0, 64, 1) WALK_REGION(1, 64, 128, 2)
+                             WALK_REGION(2, 128, 192, 3) WALK_REGION(3, 192, 256, 4)
+                                 WALK_REGION(4, 256, 320, 5) WALK_REGION(5, 320, 384, 6)
+                                     WALK_REGION(6, 384, 448, 7) WALK_REGION(7, 448, 512, 8)
+                                         WALK_END(8)
+                         WALK_OPERANDS
+                         : [lk0] "v"(link[0]), [lk1] "v"(link[1]), [lk2] "v"(link[2]),
+                           [lk3] "v"(link[3]), [lk4] "v"(link[4]), [lk5] "v"(link[5]),
+                           [lk6] "v"(link[6]), [lk7] "v"(link[7]), [fmax] "s"(kFastMax)
+                         : "scc");
+        } else if constexpr (kC == 7) {
+            asm volatile(WALK_BEGIN WALK_REGION(0, 0, 64, 1) WALK_REGION(1, 64, 128, 2)
+                             WALK_REGION(2, 128, 192, 3) WALK_REGION(3, 192, 256, 4)
+                                 WALK_REGION(4, 256, 320, 5) WALK_REGION(5, 320, 384, 6)
+                                     WALK_REGION(6, 384, 448, 7) WALK_END(7)
+                         WALK_OPERANDS
+                         : [lk0] "v"(link[0]), [lk1] "v"(link[1]), [lk2] "v"(link[2]),
+                           [lk3] "v"(link[3]), [lk4] "v"(link[4]), [lk5] "v"(link[5]),
+                           [lk6] "v"(link[6]), [fmax] "s"(kFastMax)
+                         : "scc");
+        } else {
+            static_assert(kC == 7, "walk regions are spelled out for these widths");
+        }
+        (void)fin;
+        count_out = uniform32(count);
+        return uniform32(pos);
+    }
+}
+#undef WALK_REGION
+#undef WALK_BEGIN
+#undef WALK_END
+#undef WALK_OPERANDS
 
 /* The chain from `start`, as walk_chain decides it (members while in the
  * window and under 64; a position with a stop link is no member), by
