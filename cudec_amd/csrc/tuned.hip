@@ -1901,21 +1901,58 @@ struct alignas(16) PipeRec {
     uint64_t lit_src, lit_dst, lit_len_s, match_dst, offset_s, match_len_s, dp_after;
 };
 
+/* A block's LDS. The two-wave form keeps kS window slots (a stage buffer
+ * and a record each; slot 0's stage sits in the ring's LDS); the parser
+ * fills them in turn and the copier runs them in the same order. */
+template <uint32_t kS>
 struct alignas(16) PipeLds {
     union {
         struct {
-            PipeRing w;                 /* copier's ring, stage buffer 0, links */
-            uint8_t stage1[kStageBuf];  /* stage buffer 1 */
-            PipeRec rec[2];
-        } pipe;                         /* two-wave form */
+            PipeRing w;                        /* copier's ring, slot 0's stage */
+            uint8_t stage[kS - 1][kStageBuf];  /* the other slots' stages */
+            PipeRec rec[kS];
+        } pipe;                                /* two-wave form */
         struct {
-            WaveLds a, b;               /* one wave each, once the block splits */
+            WaveLds a, b;                      /* one wave each, once the block splits */
         } split;
     };
     uint32_t chunk_idx;
-    uint32_t fin;               /* step whose record ends the chunk */
-    uint32_t pad[2];
+    uint32_t fin;           /* lockstep: the step whose record ends the chunk */
+    uint32_t pub[kS];       /* slotted: 1 + sequence number of each slot's record */
+    uint32_t freed[kS];     /* slotted: 1 + sequence number of the record run last there */
+    uint32_t broken;        /* slotted: a wait ran out, and no wave waits any more */
+
+    __device__ uint8_t* stage_of(uint32_t slot) {
+        return slot == 0 ? pipe.w.stage : pipe.stage[slot - 1];
+    }
 };
+
+/* Slotted blocks signal through LDS words, each written by one wave and
+ * polled by the other (s_sleep between reads). A wait longer than any
+ * window could take (2^27 polls: seconds) marks the block broken, and a
+ * broken block waits no more (false). */
+typedef __attribute__((address_space(3))) volatile uint32_t lds_flag;
+__device__ __forceinline__ bool await_flag(const lds_flag* flag, uint32_t want, lds_flag* broken) {
+    for (uint32_t i = 0;; i++) {
+        if (uniform32(*flag) == want) {
+            break;
+        }
+        if (uniform32(*broken) != 0) {
+            return false;
+        }
+        if (i == (1u << 27)) {
+            *broken = 1;
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return true;
+}
+__device__ __forceinline__ void raise_flag(lds_flag* flag, uint32_t value) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    *flag = value;
+}
 
 static_assert(kLaneLit < 65536 && kBatchMaxPipe < 65536 && kStageBuf < 65536,
               "window records hold 16-bit fields");
@@ -1971,6 +2008,103 @@ __device__ __forceinline__ ChunkOut decode_chunk(WaveLds& w, gcu8* src,
     r.status = status;
     r.produced = dp;
     return r;
+}
+
+/* The parser's side of one window of the two-wave form: parse (or hand
+ * over the chunk's failure) into record r and stage buffer `stage`;
+ * returns whether this record ends the chunk. */
+template <int kFormat>
+__device__ __forceinline__ bool parse_into(PipeRec& r, uint8_t* stage, int begun, uint64_t& fuel,
+                                           gcu8* src, uint64_t src_size, uint64_t cap,
+                                           uint64_t declared, uint64_t& sp, uint64_t& dp,
+                                           uint32_t lane, uint32_t& touch) {
+    Parsed pr;
+    if (begun != kOk || fuel-- == 0) {
+        pr.kind = kParseFail;
+        pr.status = begun != kOk ? begun : kCorrupt;
+        pr.done = false;
+        pr.delta = 0;
+    } else {
+        constexpr uint32_t kC = kFormat == kFormatSnappy ? kCandPipeSnappy : kCandPipe;
+        pr = parse_any<kFormat, kC, false>(stage, nullptr, src, src_size, cap, declared, sp, dp,
+                                           lane, touch);
+    }
+    r.kind = pr.kind;
+    r.status = pr.status;
+    r.done = pr.done ? 1u : 0u;
+    r.lead = pr.lead ? 1u : 0u;
+    if (pr.kind == kParseSlow || pr.lead) {
+        r.lit_src = pr.q.lit_src;
+        r.lit_dst = pr.q.lit_dst;
+        r.lit_len_s = pr.q.lit_len;
+        r.match_dst = pr.q.match_dst;
+        r.offset_s = pr.q.offset;
+        r.match_len_s = pr.q.match_len;
+        r.dp_after = pr.lead ? pr.lead_dp : dp;
+    }
+    if (pr.kind == kParseGroup) {
+        r.lit[lane] = (uint16_t)pr.g.lit;
+        r.lit_len[lane] = (uint16_t)pr.g.lit_len;
+        r.match_len[lane] = (uint16_t)pr.g.match_len;
+        r.offset[lane] = pr.g.offset;
+        r.drel[lane] = (uint16_t)pr.g.drel;
+        r.delta = pr.delta;
+        r.take = pr.g.take;
+        r.terminal = pr.g.terminal ? 1u : 0u;
+        r.end_rel = pr.g.end_rel;
+    }
+    return pr.kind == kParseFail || pr.done;
+}
+
+/* The copier's side: run record r, whose window is staged in `stage`
+ * (kParseOnly: only the output size advances); a failed record sets
+ * status. Returns whether this record ends the chunk. */
+template <bool kParseOnly>
+__device__ __forceinline__ bool run_record(PipeRing& w, const PipeRec& r, const uint8_t* stage,
+                                           gcu8* src, gu8* dst, uint64_t cap, uint64_t& dp,
+                                           uint64_t& flushed, uint64_t& ring_lo, int& status,
+                                           uint32_t lane) {
+    const uint32_t kind = uniform32(r.kind);
+    if (kind == kParseFail) {
+        status = (int)uniform32((uint32_t)r.status);
+        return true;
+    }
+    const bool done = uniform32(r.done) != 0;
+    if (kParseOnly) {
+        /* the parse ceiling: the records' sizes, no bytes */
+        if (kind == kParseSlow || uniform32(r.lead) != 0) {
+            dp = uniform64(r.dp_after);
+        }
+        if (kind == kParseGroup) {
+            dp += uniform32(r.end_rel);
+        }
+        return done;
+    }
+    if (kind == kParseSlow || uniform32(r.lead) != 0) {
+        Lz4Step s;
+        s.lit_src = uniform64(r.lit_src);
+        s.lit_dst = uniform64(r.lit_dst);
+        s.lit_len = uniform64(r.lit_len_s);
+        s.match_dst = uniform64(r.match_dst);
+        s.offset = (uint32_t)uniform64(r.offset_s);
+        s.match_len = uniform64(r.match_len_s);
+        execute_slow(w, src, dst, cap, s, uniform64(r.dp_after), dp, flushed, ring_lo,
+                     kind == kParseSlow && done, lane);
+    }
+    if (kind == kParseGroup) {
+        Group g;
+        g.lit = r.lit[lane];
+        g.lit_len = r.lit_len[lane];
+        g.match_len = r.match_len[lane];
+        g.offset = r.offset[lane];
+        g.drel = r.drel[lane];
+        g.take = uniform32(r.take);
+        g.terminal = uniform32(r.terminal) != 0;
+        g.end_rel = uniform32(r.end_rel);
+        execute_group(w, stage + uniform32(r.delta), g, dst, cap, dp, flushed, ring_lo, done,
+                      lane);
+    }
+    return done;
 }
 
 /* ------------------------------------------------------------ kernels */
@@ -2108,15 +2242,18 @@ __device__ __forceinline__ uint32_t queue_pop(Queue* q, size_t n, QueueCursor& c
 constexpr int kPipeBlock = 2 * kWave;
 constexpr uint32_t kNoFin = 0xFFFFFFFFu;
 
-/* Two waves per chunk. Wave 0 parses window i into record i % 2 (and
- * stage buffer i % 2) while wave 1 executes record i - 1; one barrier
- * separates the steps. Parsing reads only the source and decides every
- * status; execution only writes the ring and the destination, so the
+/* Two waves per chunk. Wave 0 parses windows into the block's window
+ * slots (a record and a stage buffer each) while wave 1 executes the
+ * records in the same order. Parsing reads only the source and decides
+ * every status; execution only writes the ring and the destination, so the
  * chunk's critical path is the longer of the two per window rather than
- * their sum. Control flow around the barriers is uniform for the block:
- * both waves read the chunk index and the final step from LDS after a
- * barrier, and a barrier opens every chunk so that no control word is
- * rewritten while the other wave may still read it. */
+ * their sum. Slotted blocks (four slots) hand windows over through LDS
+ * flags, so the parser may run three windows ahead; lockstep blocks (two
+ * slots) take one barrier per window. Control flow around the barriers is
+ * uniform for the block: both waves read the chunk index (and, lockstep,
+ * the final step) from LDS after a barrier, and a barrier opens every
+ * chunk so that no control word is rewritten while the other wave may
+ * still read it. */
 /* kWaves: waves per SIMD the register budget is cut for. 4 (128 VGPRs) for
  * launches that pipeline chunks; 5 (96 VGPRs, some spills on the two-wave
  * path) for launches where every chunk runs on one wave, which are
@@ -2128,11 +2265,24 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
     piped_batch(const void* const* srcs, const size_t* src_sizes, void* const* dsts,
                 const size_t* caps, size_t n, cudec_chunk_result* results, Queue* q,
                 const uint32_t* order, Plan plan) {
-    __shared__ PipeLds L;
+    /* Slotted (four waves per SIMD): four window slots signalled through
+     * LDS flags, so the parser runs up to three windows ahead and neither
+     * wave waits on the other's slowest windows (a window's parse and copy
+     * times vary independently). Lockstep (five waves per SIMD, where ten
+     * blocks must fit a CU's LDS): two slots and a barrier per window. */
+    constexpr bool kSlotted = kWaves == 4;
+    constexpr uint32_t kS = kSlotted ? 4 : 2;
+    __shared__ PipeLds<kS> L;
     const uint32_t lane = __lane_id();
     const bool parser = uniform32(threadIdx.x / kWave) == 0;
     uint32_t split_idx = 0xFFFFFFFFu; /* first chunk decoded one wave per chunk */
     QueueCursor cursor{blockIdx.x % kSubs, kSubs};
+    if (kSlotted && parser && lane < kS) {
+        L.pub[lane] = 0;
+        L.freed[lane] = 0;
+        L.broken = 0;
+    }
+    uint32_t seq = 0; /* slotted: windows of this block so far, counted alike by both waves */
     for (;;) {
         /* both waves are done reading the previous chunk's control words
          * (chunk_idx, fin) before the parser writes the next chunk's */
@@ -2160,107 +2310,76 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
         uint64_t sp = 0, declared = 0, dp = 0, flushed = 0, ring_lo = 0;
         uint64_t fuel = src_size == UINT64_MAX ? UINT64_MAX : src_size + 2;
         int status = kOk;
-        bool pdone = false;
         /* the parser's preamble verdict (Snappy), handed over as a failed
          * window record when it is not OK */
         const int begun = parser ? chunk_begin<kFormat>(src, src_size, cap, sp, declared) : kOk;
-        for (uint32_t step = 0;; step++) {
-            const uint32_t b = step & 1;
+        if constexpr (kSlotted) {
+            lds_flag* broken = (lds_flag*)&L.broken;
             if (parser) {
-                if (!pdone) {
-                    PipeRec& r = L.pipe.rec[b];
-                    Parsed pr;
+                /* each window: wait until its slot's previous record has
+                 * run, parse into the slot, publish */
+                for (;;) {
+                    const uint32_t sl = seq % kS;
+                    if (seq >= kS &&
+                        !await_flag((const lds_flag*)&L.freed[sl], seq - kS + 1, broken)) {
+                        break;
+                    }
                     uint32_t touch = 0;
-                    if (begun != kOk || fuel-- == 0) {
-                        pr.kind = kParseFail;
-                        pr.status = begun != kOk ? begun : kCorrupt;
-                        pr.done = false;
-                        pr.delta = 0;
-                    } else {
-                        constexpr uint32_t kC =
-                            kFormat == kFormatSnappy ? kCandPipeSnappy : kCandPipe;
-                        pr = parse_any<kFormat, kC, false>(
-                            b ? L.pipe.stage1 : L.pipe.w.stage, nullptr, src, src_size, cap,
-                            declared, sp, dp, lane, touch);
-                    }
-                    r.kind = pr.kind;
-                    r.status = pr.status;
-                    r.done = pr.done ? 1u : 0u;
-                    r.lead = pr.lead ? 1u : 0u;
-                    if (pr.kind == kParseSlow || pr.lead) {
-                        r.lit_src = pr.q.lit_src;
-                        r.lit_dst = pr.q.lit_dst;
-                        r.lit_len_s = pr.q.lit_len;
-                        r.match_dst = pr.q.match_dst;
-                        r.offset_s = pr.q.offset;
-                        r.match_len_s = pr.q.match_len;
-                        r.dp_after = pr.lead ? pr.lead_dp : dp;
-                    }
-                    if (pr.kind == kParseGroup) {
-                        r.lit[lane] = (uint16_t)pr.g.lit;
-                        r.lit_len[lane] = (uint16_t)pr.g.lit_len;
-                        r.match_len[lane] = (uint16_t)pr.g.match_len;
-                        r.offset[lane] = pr.g.offset;
-                        r.drel[lane] = (uint16_t)pr.g.drel;
-                        r.delta = pr.delta;
-                        r.take = pr.g.take;
-                        r.terminal = pr.g.terminal ? 1u : 0u;
-                        r.end_rel = pr.g.end_rel;
-                    }
-                    pdone = pr.kind == kParseFail || pr.done;
-                    if (pdone) {
-                        L.fin = step;
-                    }
+                    const bool last =
+                        parse_into<kFormat>(L.pipe.rec[sl], L.stage_of(sl), begun, fuel, src,
+                                            src_size, cap, declared, sp, dp, lane, touch);
+                    raise_flag((lds_flag*)&L.pub[sl], seq + 1);
+                    seq++;
                     /* the look-ahead loads land by now */
                     asm volatile("" ::"v"(touch));
+                    if (last) {
+                        break;
+                    }
                 }
-            } else if (step > 0 && status == kOk) {
-                const PipeRec& r = L.pipe.rec[b ^ 1];
-                const uint32_t kind = uniform32(r.kind);
-                if (kind == kParseFail) {
-                    status = (int)uniform32((uint32_t)r.status);
-                } else if (kParseOnly) {
-                    /* the parse ceiling: the records' sizes, no bytes */
-                    if (kind == kParseSlow || uniform32(r.lead) != 0) {
-                        dp = uniform64(r.dp_after);
+            } else {
+                /* each window: wait for its record, run it, free the slot */
+                for (;;) {
+                    const uint32_t sl = seq % kS;
+                    if (!await_flag((const lds_flag*)&L.pub[sl], seq + 1, broken)) {
+                        status = CUDEC_ERR_CUDA;
+                        break;
                     }
-                    if (kind == kParseGroup) {
-                        dp += uniform32(r.end_rel);
-                    }
-                } else {
-                    if (kind == kParseSlow || uniform32(r.lead) != 0) {
-                        Lz4Step s;
-                        s.lit_src = uniform64(r.lit_src);
-                        s.lit_dst = uniform64(r.lit_dst);
-                        s.lit_len = uniform64(r.lit_len_s);
-                        s.match_dst = uniform64(r.match_dst);
-                        s.offset = (uint32_t)uniform64(r.offset_s);
-                        s.match_len = uniform64(r.match_len_s);
-                        execute_slow(L.pipe.w, src, dst, cap, s, uniform64(r.dp_after), dp, flushed,
-                                     ring_lo, kind == kParseSlow && uniform32(r.done) != 0,
-                                     lane);
-                    }
-                    if (kind == kParseGroup) {
-                        Group g;
-                        g.lit = r.lit[lane];
-                        g.lit_len = r.lit_len[lane];
-                        g.match_len = r.match_len[lane];
-                        g.offset = r.offset[lane];
-                        g.drel = r.drel[lane];
-                        g.take = uniform32(r.take);
-                        g.terminal = uniform32(r.terminal) != 0;
-                        g.end_rel = uniform32(r.end_rel);
-                        const uint8_t* cw =
-                            (b ? L.pipe.w.stage : L.pipe.stage1) + uniform32(r.delta);
-                        execute_group(L.pipe.w, cw, g, dst, cap, dp, flushed, ring_lo,
-                                      uniform32(r.done) != 0, lane);
+                    const bool last =
+                        run_record<kParseOnly>(L.pipe.w, L.pipe.rec[sl], L.stage_of(sl), src, dst,
+                                               cap, dp, flushed, ring_lo, status, lane);
+                    raise_flag((lds_flag*)&L.freed[sl], seq + 1);
+                    seq++;
+                    if (last) {
+                        break;
                     }
                 }
             }
-            __syncthreads();
-            const uint32_t fin = uniform32(L.fin);
-            if (fin != kNoFin && step == fin + 1) {
-                break;
+        } else {
+            bool pdone = false;
+            /* step k: the parser fills slot k mod 2 while the copier runs
+             * step k - 1's record; one barrier a step */
+            for (uint32_t step = 0;; step++) {
+                const uint32_t b = step & 1;
+                if (parser) {
+                    if (!pdone) {
+                        uint32_t touch = 0;
+                        pdone = parse_into<kFormat>(L.pipe.rec[b], L.stage_of(b), begun, fuel, src,
+                                                    src_size, cap, declared, sp, dp, lane, touch);
+                        if (pdone) {
+                            L.fin = step;
+                        }
+                        /* the look-ahead loads land by now */
+                        asm volatile("" ::"v"(touch));
+                    }
+                } else if (step > 0 && status == kOk) {
+                    run_record<kParseOnly>(L.pipe.w, L.pipe.rec[b ^ 1], L.stage_of(b ^ 1), src,
+                                           dst, cap, dp, flushed, ring_lo, status, lane);
+                }
+                __syncthreads();
+                const uint32_t fin = uniform32(L.fin);
+                if (fin != kNoFin && step == fin + 1) {
+                    break;
+                }
             }
         }
         if (!parser) {
