@@ -2064,23 +2064,37 @@ __device__ __forceinline__ bool run_record(PipeRing& w, const PipeRec& r, const 
                                            gcu8* src, gu8* dst, uint64_t cap, uint64_t& dp,
                                            uint64_t& flushed, uint64_t& ring_lo, int& status,
                                            uint32_t lane) {
+    /* every field read at once, before the kind decides which are used
+     * (one LDS round trip; a slow or failed record's group fields are
+     * stale and unused) */
+    Group g;
+    g.lit = r.lit[lane];
+    g.lit_len = r.lit_len[lane];
+    g.match_len = r.match_len[lane];
+    g.offset = r.offset[lane];
+    g.drel = r.drel[lane];
+    g.take = uniform32(r.take);
+    g.terminal = uniform32(r.terminal) != 0;
+    g.end_rel = uniform32(r.end_rel);
+    const uint32_t delta = uniform32(r.delta);
     const uint32_t kind = uniform32(r.kind);
+    const bool done = uniform32(r.done) != 0;
+    const bool lead = uniform32(r.lead) != 0;
     if (kind == kParseFail) {
         status = (int)uniform32((uint32_t)r.status);
         return true;
     }
-    const bool done = uniform32(r.done) != 0;
     if (kParseOnly) {
         /* the parse ceiling: the records' sizes, no bytes */
-        if (kind == kParseSlow || uniform32(r.lead) != 0) {
+        if (kind == kParseSlow || lead) {
             dp = uniform64(r.dp_after);
         }
         if (kind == kParseGroup) {
-            dp += uniform32(r.end_rel);
+            dp += g.end_rel;
         }
         return done;
     }
-    if (kind == kParseSlow || uniform32(r.lead) != 0) {
+    if (kind == kParseSlow || lead) {
         Lz4Step s;
         s.lit_src = uniform64(r.lit_src);
         s.lit_dst = uniform64(r.lit_dst);
@@ -2092,17 +2106,7 @@ __device__ __forceinline__ bool run_record(PipeRing& w, const PipeRec& r, const 
                      kind == kParseSlow && done, lane);
     }
     if (kind == kParseGroup) {
-        Group g;
-        g.lit = r.lit[lane];
-        g.lit_len = r.lit_len[lane];
-        g.match_len = r.match_len[lane];
-        g.offset = r.offset[lane];
-        g.drel = r.drel[lane];
-        g.take = uniform32(r.take);
-        g.terminal = uniform32(r.terminal) != 0;
-        g.end_rel = uniform32(r.end_rel);
-        execute_group(w, stage + uniform32(r.delta), g, dst, cap, dp, flushed, ring_lo, done,
-                      lane);
+        execute_group(w, stage + delta, g, dst, cap, dp, flushed, ring_lo, done, lane);
     }
     return done;
 }
