@@ -188,42 +188,6 @@ __device__ __forceinline__ uint32_t stage_into(uint8_t* stage, gcu8* src,
     return (uint32_t)(a - base);
 }
 
-/* A window's stage held in registers (the two 16-byte blocks a lane of
- * stage_into, for a window of kCandPipe candidates), loaded while the
- * previous window is handed over and written to LDS once the window's slot
- * is free, so the parser does not wait for the load. sp: the window start
- * it holds (UINT64_MAX: none). */
-struct StageAhead {
-    uint4 v0, v1;
-    uint64_t sp;
-};
-static_assert(stage_bytes(kCandPipe) / 16 + 1 <= 2 * kWave, "two blocks a lane stage a window");
-__device__ __forceinline__ uint4 stage_block(gcu8* src, uint64_t src_size, uintptr_t base,
-                                             uint32_t blk) {
-    const uintptr_t g = base + 16u * blk;
-    const uintptr_t lo = (uintptr_t)src;
-    return g < lo + src_size && g + 16 > lo ? gload16((gcu8*)g) : make_uint4(0, 0, 0, 0);
-}
-__device__ __forceinline__ void stage_load(StageAhead& a, gcu8* src, uint64_t src_size,
-                                           uint64_t sp, uint32_t lane) {
-    const uintptr_t base = (uintptr_t)(src + sp) & ~(uintptr_t)15;
-    constexpr uint32_t kBlocks = stage_bytes(kCandPipe) / 16 + 1;
-    a.v0 = stage_block(src, src_size, base, lane);
-    a.v1 = lane + kWave < kBlocks ? stage_block(src, src_size, base, lane + kWave)
-                                  : make_uint4(0, 0, 0, 0);
-    a.sp = sp;
-}
-__device__ __forceinline__ uint32_t stage_store(uint8_t* stage, const StageAhead& a, gcu8* src,
-                                                uint32_t lane) {
-    constexpr uint32_t kBlocks = stage_bytes(kCandPipe) / 16 + 1;
-    *reinterpret_cast<uint4*>(&stage[16u * lane]) = a.v0;
-    if (lane + kWave < kBlocks) {
-        *reinterpret_cast<uint4*>(&stage[16u * (lane + kWave)]) = a.v1;
-    }
-    wave_order();
-    return (uint32_t)((uintptr_t)(src + a.sp) & 15);
-}
-
 /* Loads one dword per lane of the 1 KiB after the window staged at sp
  * (16 bytes apart, every one inside the source), so that the next window's
  * stage finds those bytes in L2. The caller holds the value until the
@@ -1348,8 +1312,7 @@ template <uint32_t kC, bool kDoubling>
 __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, gcu8* src,
                                                uint64_t src_size, uint64_t cap, uint64_t& sp,
                                                uint64_t& dp, uint32_t lane,
-                                               uint32_t& touch,
-                                               const StageAhead* ahead = nullptr) {
+                                               uint32_t& touch) {
     Parsed out;
     out.kind = kParseFail;
     out.status = kOk;
@@ -1357,11 +1320,7 @@ __device__ __forceinline__ Parsed parse_window(uint8_t* stage, uint16_t* jump, g
     out.lead = false;
     out.lead_dp = 0;
     const uint64_t base = sp; /* window start; member positions are relative to it */
-    if (kC == kCandPipe && ahead != nullptr && uniform64(ahead->sp) == sp) {
-        out.delta = stage_store(stage, *ahead, src, lane);
-    } else {
-        out.delta = stage_into<kC>(stage, src, src_size, sp, lane);
-    }
+    out.delta = stage_into<kC>(stage, src, src_size, sp, lane);
     touch = touch_ahead<kC>(src, src_size, sp, lane);
     const uint8_t* cw = stage + out.delta;
     /* the scalar ladder reads a sequence's header from the stage while it
@@ -1828,13 +1787,12 @@ __device__ __forceinline__ Parsed parse_any(uint8_t* stage, uint16_t* jump, gcu8
                                             uint64_t src_size,
                                             uint64_t cap, uint64_t declared, uint64_t& sp,
                                             uint64_t& dp, uint32_t lane,
-                                            uint32_t& touch, const StageAhead* ahead = nullptr) {
+                                            uint32_t& touch) {
     if (kFormat == kFormatSnappy) {
         return parse_window_snappy<kC, kDoubling>(stage, jump, src, src_size, cap, declared, sp,
                                                   dp, lane, touch);
     }
-    return parse_window<kC, kDoubling>(stage, jump, src, src_size, cap, sp, dp, lane, touch,
-                                       ahead);
+    return parse_window<kC, kDoubling>(stage, jump, src, src_size, cap, sp, dp, lane, touch);
 }
 
 /* Start of a chunk: the Snappy preamble and its declared length against the
@@ -2062,8 +2020,7 @@ template <int kFormat>
 __device__ __forceinline__ bool parse_into(PipeRec& r, uint8_t* stage, int begun, uint64_t& fuel,
                                            gcu8* src, uint64_t src_size, uint64_t cap,
                                            uint64_t declared, uint64_t& sp, uint64_t& dp,
-                                           uint32_t lane, uint32_t& touch,
-                                           const StageAhead* ahead = nullptr) {
+                                           uint32_t lane, uint32_t& touch) {
     Parsed pr;
     if (begun != kOk || fuel-- == 0) {
         pr.kind = kParseFail;
@@ -2073,7 +2030,7 @@ __device__ __forceinline__ bool parse_into(PipeRec& r, uint8_t* stage, int begun
     } else {
         constexpr uint32_t kC = kFormat == kFormatSnappy ? kCandPipeSnappy : kCandPipe;
         pr = parse_any<kFormat, kC, false>(stage, nullptr, src, src_size, cap, declared, sp, dp,
-                                           lane, touch, ahead);
+                                           lane, touch);
     }
     r.kind = pr.kind;
     r.status = pr.status;
@@ -2368,8 +2325,6 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
             if (parser) {
                 /* each window: wait until its slot's previous record has
                  * run, parse into the slot, publish */
-                StageAhead ahead;
-                ahead.sp = UINT64_MAX;
                 for (;;) {
                     const uint32_t sl = seq % kS;
                     if (seq >= kS &&
@@ -2379,15 +2334,9 @@ __global__ void __launch_bounds__(kPipeBlock) __attribute__((amdgpu_waves_per_eu
                     uint32_t touch = 0;
                     const bool last =
                         parse_into<kFormat>(L.pipe.rec[sl], L.stage_of(sl), begun, fuel, src,
-                                            src_size, cap, declared, sp, dp, lane, touch,
-                                            kFormat == kFormatLz4 ? &ahead : nullptr);
+                                            src_size, cap, declared, sp, dp, lane, touch);
                     raise_flag((lds_flag*)&L.pub[sl], seq + 1);
                     seq++;
-                    if (kFormat == kFormatLz4 && !last) {
-                        /* the next window's stage, loaded while the slot
-                         * it goes to may still be busy */
-                        stage_load(ahead, src, src_size, sp, lane);
-                    }
                     /* the look-ahead loads land by now */
                     asm volatile("" ::"v"(touch));
                     if (last) {
