@@ -996,16 +996,14 @@ __device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC
          * instructions separate a v_readlane from the one that uses its
          * result as the select, the 4 wait states that use needs. */
         uint32_t prev, nx, t;
-        /* The count lives in m0 during the loop (saved and restored around
-         * it): v_writelane takes its value and lane select from scalar
+        /* The count is an operand bound to m0 ("{m0}"), so the compiler
+         * moves it in and out and keeps its own uses of m0 apart:
+         * v_writelane takes its value and lane select from scalar
          * registers, and only m0 as the lane select keeps it within one
          * constant-bus read. The leading s_nop covers a lane select (rel)
          * that the compiler may have just written from a vector
          * instruction, and m0 written just before its first use. */
-        uint32_t m0_saved;
         asm volatile(
-            "s_mov_b32 %[m0s], m0\n\t"
-            "s_mov_b32 m0, %[cnt]\n\t"
             "s_nop 4\n\t"
             "s_cmp_gt_u32 %[cnt], %[fmax]\n\t"
             "s_cbranch_scc1 walk_loop_%=\n"
@@ -1043,8 +1041,6 @@ __device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC
             "s_mov_b32 %[prev], %[pos]\n\t"
             "s_mov_b32 %[pos], %[nx]\n"
             "walk_done_%=:\n\t"
-            "s_mov_b32 %[cnt], m0\n\t"
-            "s_mov_b32 m0, %[m0s]\n\t"
             /* a stop link: prev is no member, and the walk ends there */
             "s_cmpk_eq_u32 %[pos], 0xffff\n\t"
             "s_cbranch_scc0 walk_nostop_%=\n\t"
@@ -1056,9 +1052,8 @@ __device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC
             "s_cmp_eq_u32 %[cnt], 64\n\t"
             "s_cselect_b32 %[fin], 1, 0\n"
             "walk_end_%=:"
-            : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+s"(count), [rel] "+s"(rel),
-              [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [m0s] "=&s"(m0_saved),
-              [fin] "=&s"(fin)
+            : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+{m0}"(count), [rel] "+s"(rel),
+              [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [fin] "=&s"(fin)
             : [lk] "v"(link[k]), [base] "s"(k * kWave), [lim] "s"(k * kWave + kWave),
               [fmax] "s"(kFastMax)
             : "scc");
