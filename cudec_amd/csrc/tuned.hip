@@ -968,16 +968,18 @@ __device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC
      * the walk takes the unchecked loop there */
     constexpr uint32_t kRegionMax = (kWave - 1) / kMinStep + 1;
     constexpr uint32_t kFastMax = kWave - kRegionMax;
-    uint32_t pos = start, count = 0, fin = 0;
-    /* kStraight: a straight run of guarded regions with no early exit (an
-     * exit edge out of the unrolled loop has the compiler re-materialise
-     * live flags at every region; Snappy's walk gains 2% without it, LZ4's
-     * loses 0.4%) */
+    /* A straight run of guarded regions with no early exit (an exit edge
+     * out of the unrolled loop has the compiler re-materialise live values
+     * at every region), whatever kStraight says. A region is entered only
+     * while pos lies in it and fewer than 64 members are held; a stop link
+     * lies past every region, so the walk's end needs no flag of its own
+     * and is settled once, after the last region. */
+    uint32_t pos = start, count = 0, prev = kStop;
 #pragma unroll
     for (uint32_t k = 0; k < kC; k++) {
         uint32_t rel = pos - k * kWave;
-        if ((kStraight && fin != 0) || rel >= kWave) {
-            continue; /* the walk has ended, or pos lies in a later region */
+        if ((rel | count) >= kWave) {
+            continue; /* pos lies in a later region, or the walk has ended */
         }
         /* Per member: read the link of pos out of its lane, record pos as
          * member `count`, step. The loop leaves once pos is outside this
@@ -995,7 +997,7 @@ __device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC
          * checked loop tests the count as well. In the fast loop four
          * instructions separate a v_readlane from the one that uses its
          * result as the select, the 4 wait states that use needs. */
-        uint32_t prev, nx, t;
+        uint32_t nx, t;
         /* The count is an operand bound to m0 ("{m0}"), so the compiler
          * moves it in and out and keeps its own uses of m0 apart:
          * v_writelane takes its value and lane select from scalar
@@ -1040,26 +1042,19 @@ __device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC
             "walk_out_nx_%=:\n\t"
             "s_mov_b32 %[prev], %[pos]\n\t"
             "s_mov_b32 %[pos], %[nx]\n"
-            "walk_done_%=:\n\t"
-            /* a stop link: prev is no member, and the walk ends there */
-            "s_cmpk_eq_u32 %[pos], 0xffff\n\t"
-            "s_cbranch_scc0 walk_nostop_%=\n\t"
-            "s_mov_b32 %[pos], %[prev]\n\t"
-            "s_sub_u32 %[cnt], %[cnt], 1\n\t"
-            "s_mov_b32 %[fin], 1\n\t"
-            "s_branch walk_end_%=\n"
-            "walk_nostop_%=:\n\t"
-            "s_cmp_eq_u32 %[cnt], 64\n\t"
-            "s_cselect_b32 %[fin], 1, 0\n"
-            "walk_end_%=:"
+            "walk_done_%=:"
             : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+{m0}"(count), [rel] "+s"(rel),
-              [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [fin] "=&s"(fin)
+              [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t)
             : [lk] "v"(link[k]), [base] "s"(k * kWave), [lim] "s"(k * kWave + kWave),
               [fmax] "s"(kFastMax)
             : "scc");
-        if (!kStraight && fin != 0) {
-            break; /* a stop link, or 64 members */
-        }
+    }
+    /* a stop link: the member that read it (prev, the last member) is no
+     * member, and the walk ends there; a walk that started on a stop link
+     * holds no member and stays as it is */
+    if (pos == kStop && count != 0) {
+        pos = prev;
+        count -= 1;
     }
     count_out = uniform32(count);
     return uniform32(pos);
