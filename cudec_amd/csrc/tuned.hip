@@ -1071,14 +1071,17 @@ __device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC
  * uses bits [5:0] of the select), else the checked loop, which tests the
  * count as well. Four instructions separate a v_readlane from the one that
  * uses its result as the select (the 4 wait states that use needs), and
- * three separate an m0 update from the v_writelane that reads it. On
- * leaving the region (pos = the next position): a stop link or 64 members
- * end the walk, else the next region's check follows. */
+ * three separate an m0 update from the v_writelane that reads it. A
+ * region is entered only while pos lies in it and fewer than 64 members are
+ * held ((pos - base) | count below 64); leaving it (pos = the next position)
+ * falls into the next region's check. A stop link lies past every region,
+ * so the walk's end is settled once, in WALK_END. */
 #define WALK_REGION(k, B, L, N)                                                             \
     "r" #k "_%=:\n\t"                                                                       \
-    "s_cmp_lt_u32 %[pos], " #L "\n\t"                                                       \
-    "s_cbranch_scc0 r" #N "_%=\n\t"                                                         \
     "s_sub_u32 %[rel], %[pos], " #B "\n\t"                                                  \
+    "s_or_b32 %[t], %[rel], m0\n\t"                                                         \
+    "s_cmp_lt_u32 %[t], 64\n\t"                                                             \
+    "s_cbranch_scc0 r" #N "_%=\n\t"                                                         \
     "s_cmp_gt_u32 m0, %[fmax]\n\t"                                                          \
     "s_cbranch_scc1 c" #k "_%=\n"                                                           \
     "f" #k "_%=:\n\t"                                                                       \
@@ -1114,42 +1117,35 @@ __device__ __forceinline__ uint32_t walk_chain_regions(const uint32_t (&link)[kC
     "o" #k "_%=:\n\t"                                                                       \
     "s_mov_b32 %[prev], %[pos]\n\t"                                                         \
     "s_mov_b32 %[pos], %[nx]\n"                                                             \
-    "x" #k "_%=:\n\t"                                                                       \
-    "s_cmpk_eq_u32 %[pos], 0xffff\n\t"                                                      \
-    "s_cbranch_scc1 stop_%=\n\t"                                                            \
-    "s_cmp_eq_u32 m0, 64\n\t"                                                               \
-    "s_cbranch_scc1 full_%=\n\t"                                                            \
-    "s_branch r" #N "_%=\n"
+    "x" #k "_%=:\n"
 
 /* The walk's entry and exits around the regions: the count lives in m0
  * during the walk (saved and restored around it): v_writelane takes its
  * value and lane select from scalar registers, and only m0 as the lane
  * select keeps it within one constant-bus read. The leading s_nop covers a
  * lane select the compiler may have just written from a vector instruction,
- * and m0 written just before its first use. A stop link: prev is no
- * member, and the walk ends there; past the last region: the window ends. */
+ * and m0 written just before its first use. Past the last region: a stop
+ * link (pos = 0xffff) read by the last member (prev) makes it no member, and
+ * the walk ends there (a walk that started on a stop link holds none);
+ * otherwise the walk ended past the window or at 64 members. */
 #define WALK_BEGIN                                                                          \
     "s_mov_b32 %[m0s], m0\n\t"                                                              \
     "s_mov_b32 m0, %[cnt]\n\t"                                                              \
     "s_nop 4\n"
 #define WALK_END(N)                                                                         \
     "r" #N "_%=:\n\t"                                                                       \
-    "s_mov_b32 %[fin], 0\n\t"                                                               \
-    "s_branch end_%=\n"                                                                     \
-    "stop_%=:\n\t"                                                                          \
+    "s_cmpk_eq_u32 %[pos], 0xffff\n\t"                                                      \
+    "s_cbranch_scc0 end_%=\n\t"                                                             \
+    "s_cmp_eq_u32 m0, 0\n\t"                                                                \
+    "s_cbranch_scc1 end_%=\n\t"                                                             \
     "s_mov_b32 %[pos], %[prev]\n\t"                                                         \
-    "s_mov_b32 %[fin], 1\n\t"                                                               \
-    "s_sub_u32 m0, m0, 1\n\t"                                                               \
-    "s_branch end_%=\n"                                                                     \
-    "full_%=:\n\t"                                                                          \
-    "s_mov_b32 %[fin], 1\n"                                                                 \
+    "s_sub_u32 m0, m0, 1\n"                                                                 \
     "end_%=:\n\t"                                                                           \
     "s_mov_b32 %[cnt], m0\n\t"                                                              \
     "s_mov_b32 m0, %[m0s]"
 #define WALK_OPERANDS                                                                       \
     : [mem] "+v"(member), [pos] "+s"(pos), [cnt] "+s"(count), [rel] "=&s"(rel),             \
-      [prev] "=&s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [m0s] "=&s"(m0_saved),              \
-      [fin] "=&s"(fin)
+      [prev] "+s"(prev), [nx] "=&s"(nx), [t] "=&s"(t), [m0s] "=&s"(m0_saved)
 
 /* The walk of walk_chain_regions, with the same results: for windows of up
  * to eight regions as one asm block (WALK_REGION per region), else region
@@ -1169,7 +1165,7 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
          * at once and spill (5% slower on LZ4 than a block per region) */
         return walk_chain_regions<kC, kMinStep, kStraight>(link, start, count_out, member);
     } else {
-        uint32_t pos = start, count = 0, fin = 0, rel, prev = 0, nx, t, m0_saved;
+        uint32_t pos = start, count = 0, rel, prev = kStop, nx, t, m0_saved;
         /* one asm block for the whole walk: no compiler code between regions
          * (Snappy's seven regions: 2% faster than a block per region) */
         if constexpr (kC == 8) {
@@ -1196,7 +1192,6 @@ __device__ __forceinline__ uint32_t walk_chain(const uint32_t (&link)[kC], uint3
         } else {
             static_assert(kC == 7, "walk regions are spelled out for these widths");
         }
-        (void)fin;
         count_out = uniform32(count);
         return uniform32(pos);
     }
